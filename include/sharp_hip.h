@@ -1,0 +1,66 @@
+/*
+ * sharp_hip.h -- entry points of the MI355X engine beyond libsharp's
+ * interface.  None of these exists in the reference; they serve callers whose
+ * data already lives in GPU memory (the reference equivalent of each one is
+ * named in its comment).
+ *
+ * Conventions: `alm` and `map` are host arrays of DEVICE pointers with the
+ * same meaning and count as in sharp_execute (libsharp/sharp_lowlevel.h:
+ * 196-224); `stream` is a hipStream_t (NULL: the library's own stream on the
+ * current device).  The call enqueues the work and returns; the caller
+ * synchronises the stream before reading the outputs.  Argument errors
+ * behave as in sharp_execute (message, then exit(1)).
+ */
+#ifndef SHARP_HIP_H
+#define SHARP_HIP_H
+
+#include "sharp_lowlevel.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision of the Legendre stage, or-ed into `flags` (bits libsharp does not
+   use).  Default: FP64 throughout.  The mixed forms keep the lambda_lm
+   recurrence, its start values and its exponent scaling in FP64 and run the
+   [rings x l] . [l x maps] contraction on the matrix cores. */
+#define SHARP_HIP_MIXED_FP32    (1<<24)   /* FP32 MFMA, FP32 accumulation  */
+#define SHARP_HIP_MIXED_BF16X3  (1<<25)   /* split-BF16 MFMA, FP32 accum.  */
+
+/* Device-resident form of sharp_execute (libsharp/sharp_lowlevel.h:222-224).
+   *opcnt as in sharp_execute; no timing is taken. */
+void sharp_hip_execute_device (sharp_jobtype type, int spin, void *alm,
+  void *map, const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+  int ntrans, int flags, void *stream, unsigned long long *opcnt);
+
+/* Adjoint of synthesis, x -> Y^T x: map2alm without quadrature weights
+   (sharp_execute with SHARP_Yt, libsharp/sharp.c:848). */
+void sharp_hip_alm2map_adjoint_device (int spin, void *alm, void *map,
+  const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+  int ntrans, int flags, void *stream);
+
+/* Adjoint of analysis, a -> W Y a: alm2map with quadrature weights
+   (sharp_execute with SHARP_WY, libsharp/sharp.c:849). */
+void sharp_hip_map2alm_adjoint_device (int spin, void *alm, void *map,
+  const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+  int ntrans, int flags, void *stream);
+
+/* Builds (and caches) the device tables for this geometry / a_lm layout /
+   spin outside any timed region.  Optional: the first transform does it. */
+void sharp_hip_prepare (const sharp_geom_info *geom_info,
+  const sharp_alm_info *alm_info, int spin);
+
+/* Waits for all work the library enqueued on `stream` (NULL: its own). */
+void sharp_hip_synchronize (void *stream);
+
+/* Releases every cached table and work buffer of the current device. */
+void sharp_hip_release (void);
+
+/* Number of visible GPUs (0 on a host without one). */
+int sharp_hip_device_count (void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

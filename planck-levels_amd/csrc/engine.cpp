@@ -1,0 +1,704 @@
+// engine.cpp -- host side of the MI355X SHT engine: job normalisation and
+// validation, per-device caches of geometry / recurrence tables, host<->device
+// staging for the libsharp ABI, and the sequence of kernel launches.
+//
+// Job normalisation and checks follow libsharp/sharp.c:238-251,762-880 and
+// libsharp/sharp_ylmgen_c.c:44-52; the recurrence tables are built exactly as
+// libsharp/sharp_ylmgen_c.c:44-232 builds them (same operations in the same
+// order), so both implementations start from bit-identical coefficients.
+//
+// Pipeline per call (one HIP stream, no host synchronisation inside):
+//   synthesis: alm_gather -> Legendre synthesis (per transform group)
+//              -> phase2ring (writes the map, adds with SHARP_ADD)
+//   analysis:  ring2phase -> Legendre analysis (per transform group)
+//              -> alm_scatter (writes the a_lm, adds with SHARP_ADD)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "sharp_lowlevel.h"
+#include "sharp_hip.h"
+#include "shp_internal.hpp"
+#include "kernels.hpp"
+
+namespace shp {
+
+#define HIP_CHECK(expr)                                                        \
+  do {                                                                         \
+    const hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) {                                                    \
+      char b_[320];                                                            \
+      std::snprintf(b_, sizeof(b_), "HIP error '%s' in %s",                    \
+                    hipGetErrorString(e_), #expr);                             \
+      SHP_FAIL(b_);                                                            \
+    }                                                                          \
+  } while (0)
+
+static const double kPi = 3.141592653589793238462643383279502884197;
+
+// ---------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  void *get(size_t bytes)
+  {
+    if (bytes == 0) bytes = 16;
+    if (bytes > cap) {
+      if (p) HIP_CHECK(hipFree(p));
+      p = nullptr;
+      HIP_CHECK(hipMalloc(&p, bytes));
+      cap = bytes;
+    }
+    return p;
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+template <typename T> static T *upload(const std::vector<T> &v)
+{
+  T *d = nullptr;
+  HIP_CHECK(hipMalloc(&d, std::max<size_t>(v.size(), 1) * sizeof(T)));
+  if (!v.empty())
+    HIP_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// cached device objects
+// ---------------------------------------------------------------------------
+struct GeomPlan {
+  std::vector<char> key;     // the ring data this plan was built from
+  int nslots = 0, nphmax = 0;
+  std::vector<int> order;    // slot -> index into geom_info->pair
+  std::vector<double> hcth, hsth;
+  double *cth = nullptr, *sth = nullptr;
+  int *ispair = nullptr;
+  RingDev *ring = nullptr;
+  struct Mlim {
+    int lmax, spin;
+    std::vector<int> h;
+    int *d = nullptr;
+  };
+  std::vector<Mlim> mlims;
+  ~GeomPlan()
+  {
+    (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
+    for (auto &m : mlims) (void)hipFree(m.d);
+  }
+};
+
+struct YlmPlan {
+  int lmax, mmax, spin;
+  double *tab = nullptr;     // rows at tab_ofs(m, lmax)
+  double *mfac = nullptr;    // spin 0
+  double *prefac = nullptr;  // spin s
+  int *fscale = nullptr;
+  ~YlmPlan()
+  {
+    (void)hipFree(tab); (void)hipFree(mfac); (void)hipFree(prefac); (void)hipFree(fscale);
+  }
+};
+
+struct NormPlan {
+  int lmax, spin, deriv;
+  double *d = nullptr;
+  ~NormPlan() { (void)hipFree(d); }
+};
+
+struct AlmPlan {
+  std::vector<char> key;
+  int64_t *mvstart_by_m = nullptr;
+  ~AlmPlan() { (void)hipFree(mvstart_by_m); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<GeomPlan>> geoms;
+  std::vector<std::unique_ptr<YlmPlan>> ylms;
+  std::vector<std::unique_ptr<NormPlan>> norms;
+  std::vector<std::unique_ptr<AlmPlan>> alms;
+  DevBuf phase, almtmp, stage_alm, stage_map;
+};
+
+static std::mutex g_mutex;
+static std::map<int, std::unique_ptr<Ctx>> g_ctx;
+
+static Ctx &context()
+{
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    SHP_FAIL("no GPU device available: the HIP engine cannot run here");
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  auto it = g_ctx.find(dev);
+  if (it != g_ctx.end()) return *it->second;
+  auto c = std::make_unique<Ctx>();
+  c->device = dev;
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  Ctx &ref = *c;
+  g_ctx[dev] = std::move(c);
+  return ref;
+}
+
+// --- geometry ---------------------------------------------------------------
+static std::vector<char> geom_key(const sharp_geom_info *g)
+{
+  std::vector<char> k;
+  k.reserve(size_t(g->npairs) * 2 * sizeof(sharp_ringinfo) + 8);
+  auto put = [&](const void *p, size_t n) {
+    const char *c = static_cast<const char *>(p);
+    k.insert(k.end(), c, c + n);
+  };
+  put(&g->npairs, sizeof(int));
+  for (int i = 0; i < g->npairs; ++i) {
+    put(&g->pair[i].r1, sizeof(sharp_ringinfo));
+    if (g->pair[i].r2.nph > 0)
+      put(&g->pair[i].r2, sizeof(sharp_ringinfo));
+    else {
+      const int none = -1;
+      put(&none, sizeof(int));
+    }
+  }
+  return k;
+}
+
+static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
+{
+  std::vector<char> key = geom_key(g);
+  for (auto &p : c.geoms)
+    if (p->key == key) return *p;
+  auto gp = std::make_unique<GeomPlan>();
+  gp->key = std::move(key);
+  const int P = g->npairs;
+  gp->nslots = P;
+  gp->order.resize(P);
+  std::iota(gp->order.begin(), gp->order.end(), 0);
+  // device order: by sin(theta) of the northern ring, so that wavefronts see
+  // rings with similar m cutoffs and similar exponent-scaling behaviour
+  std::stable_sort(gp->order.begin(), gp->order.end(), [g](int a, int b) {
+    return g->pair[a].r1.sth < g->pair[b].r1.sth;
+  });
+  std::vector<int> ispair(P);
+  std::vector<RingDev> ring(2 * size_t(P));
+  gp->hcth.resize(P);
+  gp->hsth.resize(P);
+  int nphmax = 0;
+  for (int k = 0; k < P; ++k) {
+    const sharp_ringpair &pr = g->pair[gp->order[k]];
+    gp->hcth[k] = pr.r1.cth;
+    gp->hsth[k] = pr.r1.sth;
+    ispair[k] = pr.r2.nph > 0;
+    ring[2 * k] = RingDev{(int64_t)pr.r1.ofs, pr.r1.nph, pr.r1.stride, pr.r1.phi0,
+                          pr.r1.weight};
+    if (pr.r2.nph > 0)
+      ring[2 * k + 1] = RingDev{(int64_t)pr.r2.ofs, pr.r2.nph, pr.r2.stride,
+                                pr.r2.phi0, pr.r2.weight};
+    else
+      ring[2 * k + 1] = RingDev{0, -1, 0, 0., 0.};
+    nphmax = std::max(nphmax, std::max(pr.r1.nph, pr.r2.nph));
+  }
+  gp->nphmax = nphmax;
+  gp->cth = upload(gp->hcth);
+  gp->sth = upload(gp->hsth);
+  gp->ispair = upload(ispair);
+  gp->ring = upload(ring);
+  c.geoms.push_back(std::move(gp));
+  return *c.geoms.back();
+}
+
+static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
+{
+  for (auto &m : gp.mlims)
+    if (m.lmax == lmax && m.spin == spin) return m;
+  GeomPlan::Mlim ml;
+  ml.lmax = lmax;
+  ml.spin = spin;
+  ml.h.resize(gp.nslots);
+  for (int k = 0; k < gp.nslots; ++k)
+    ml.h[k] = sharp_get_mlim(lmax, spin, gp.hsth[k], gp.hcth[k]);
+  ml.d = upload(ml.h);
+  gp.mlims.push_back(std::move(ml));
+  return gp.mlims.back();
+}
+
+// --- recurrence tables (libsharp/sharp_ylmgen_c.c:44-198) --------------------
+static void host_normalize(double &val, int &scale, double xfmax)
+{
+  while (std::fabs(val) > xfmax) { val *= kFSmall; ++scale; }
+  if (val != 0.)
+    while (std::fabs(val) < xfmax * kFSmall) { val *= kFBig; --scale; }
+}
+
+static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
+{
+  for (auto &p : c.ylms)
+    if (p->lmax == lmax && p->mmax == mmax && p->spin == spin) return *p;
+  auto yp = std::make_unique<YlmPlan>();
+  yp->lmax = lmax;
+  yp->mmax = mmax;
+  yp->spin = spin;
+  const size_t nent = (size_t)tab_ofs(mmax + 1, lmax);
+  if (spin == 0) {
+    std::vector<double> tab(2 * nent, 0.), root(2 * lmax + 5), iroot(2 * lmax + 5);
+    for (int k = 0; k < 2 * lmax + 5; ++k) {
+      root[k] = std::sqrt((double)k);
+      iroot[k] = (k == 0) ? 0. : 1. / root[k];
+    }
+    for (int m = 0; m <= mmax; ++m) {
+      double *row = tab.data() + 2 * tab_ofs(m, lmax);
+      row[0] = root[2 * m + 3];
+      row[1] = 0.;
+      for (int l = m + 1; l <= lmax; ++l) {
+        const double tmp = root[2 * l + 3] * iroot[l + 1 + m] * iroot[l + 1 - m];
+        row[2 * (l - m)] = tmp * root[2 * l + 1];
+        row[2 * (l - m) + 1] = tmp * root[l + m] * root[l - m] * iroot[2 * l - 1];
+      }
+    }
+    std::vector<double> mfac(mmax + 1);
+    mfac[0] = 0.2820947917738781434740397257803862929220;   // 1/sqrt(4 pi)
+    for (int m = 1; m <= mmax; ++m)
+      mfac[m] = mfac[m - 1] * std::sqrt((2 * m + 1.) / (2 * m));
+    yp->tab = upload(tab);
+    yp->mfac = upload(mfac);
+  } else {
+    const int s = spin;
+    std::vector<double> tab(3 * nent, 0.);
+    std::vector<double> flm1(2 * lmax + 1), flm2(2 * lmax + 1), inv(lmax + 1);
+    inv[0] = 0;
+    for (int k = 1; k < lmax + 1; ++k) inv[k] = 1. / k;
+    for (int k = 0; k < 2 * lmax + 1; ++k) {
+      flm1[k] = std::sqrt(1. / (k + 1.));
+      flm2[k] = std::sqrt(k / (k + 1.));
+    }
+    for (int m = 0; m <= mmax; ++m) {
+      double *row = tab.data() + 3 * tab_ofs(m, lmax);
+      const int mhi = std::max(m, s);
+      for (int l = mhi; l < lmax; ++l) {
+        double t = flm1[l + m] * flm1[l - m] * flm1[l + s] * flm1[l - s];
+        const double lt = 2 * l + 1, l1 = l + 1;
+        double *e = row + 3 * (l + 1 - m);
+        e[0] = l1 * lt * t;
+        e[1] = m * s * inv[l] * inv[l + 1];
+        t = flm2[l + m] * flm2[l - m] * flm2[l + s] * flm2[l - s];
+        e[2] = t * l1 * inv[l];
+      }
+    }
+    std::vector<double> fac(2 * lmax + 1);
+    std::vector<int> facscale(2 * lmax + 1);
+    fac[0] = 1;
+    facscale[0] = 0;
+    for (int k = 1; k < 2 * lmax + 1; ++k) {
+      fac[k] = fac[k - 1] * std::sqrt((double)k);
+      facscale[k] = facscale[k - 1];
+      host_normalize(fac[k], facscale[k], kFBigHalf);
+    }
+    std::vector<double> prefac(mmax + 1);
+    std::vector<int> fscale(mmax + 1);
+    for (int m = 0; m <= mmax; ++m) {
+      int mlo = s, mhi = m;
+      if (mhi < mlo) std::swap(mhi, mlo);
+      double tfac = fac[2 * mhi] / fac[mhi + mlo];
+      int tscale = facscale[2 * mhi] - facscale[mhi + mlo];
+      host_normalize(tfac, tscale, kFBigHalf);
+      tfac /= fac[mhi - mlo];
+      tscale -= facscale[mhi - mlo];
+      host_normalize(tfac, tscale, kFBigHalf);
+      prefac[m] = tfac;
+      fscale[m] = tscale;
+    }
+    yp->tab = upload(tab);
+    yp->prefac = upload(prefac);
+    yp->fscale = upload(fscale);
+  }
+  c.ylms.push_back(std::move(yp));
+  return *c.ylms.back();
+}
+
+// norm_l (libsharp/sharp_ylmgen_c.c:200-232)
+static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
+{
+  for (auto &p : c.norms)
+    if (p->lmax == lmax && p->spin == spin && p->deriv == deriv) return p->d;
+  std::vector<double> n(lmax + 1);
+  if (deriv) {
+    for (int l = 0; l <= lmax; ++l)
+      n[l] = (l < 1) ? 0. : 0.5 * std::sqrt(l * (l + 1.) * (2 * l + 1.) / (4 * kPi));
+  } else if (spin == 0) {
+    for (int l = 0; l <= lmax; ++l) n[l] = 1.;
+  } else {
+    double spinsign = (spin > 0) ? -1.0 : 1.0;
+    spinsign = (spin & 1) ? -spinsign : spinsign;
+    for (int l = 0; l <= lmax; ++l)
+      n[l] = (l < spin) ? 0. : spinsign * 0.5 * std::sqrt((2 * l + 1) / (4 * kPi));
+  }
+  auto np = std::make_unique<NormPlan>();
+  np->lmax = lmax;
+  np->spin = spin;
+  np->deriv = deriv;
+  np->d = upload(n);
+  c.norms.push_back(std::move(np));
+  return c.norms.back()->d;
+}
+
+static int64_t *get_almplan(Ctx &c, const sharp_alm_info *a, int mmax)
+{
+  std::vector<char> key;
+  auto put = [&](const void *p, size_t n) {
+    const char *ch = static_cast<const char *>(p);
+    key.insert(key.end(), ch, ch + n);
+  };
+  put(&a->nm, sizeof(int));
+  put(a->mval, sizeof(int) * a->nm);
+  put(a->mvstart, sizeof(ptrdiff_t) * a->nm);
+  for (auto &p : c.alms)
+    if (p->key == key) return p->mvstart_by_m;
+  std::vector<int64_t> mvs(mmax + 1);
+  for (int mi = 0; mi < a->nm; ++mi) mvs[a->mval[mi]] = a->mvstart[mi];
+  auto ap = std::make_unique<AlmPlan>();
+  ap->key = std::move(key);
+  ap->mvstart_by_m = upload(mvs);
+  c.alms.push_back(std::move(ap));
+  return c.alms.back()->mvstart_by_m;
+}
+
+// ---------------------------------------------------------------------------
+// job normalisation (libsharp/sharp.c:840-867) and checks
+// ---------------------------------------------------------------------------
+struct Job {
+  int type, spin, nmaps, nalm, ntrans, flags, lmax, mmax, deriv;
+  const sharp_geom_info *g;
+  const sharp_alm_info *a;
+};
+
+static Job make_job(int type, int spin, const sharp_geom_info *g,
+                    const sharp_alm_info *a, int ntrans, int flags)
+{
+  if (type < 0 || type > 4) SHP_FAIL("bad type");
+  if (ntrans <= 0 || ntrans > 100) SHP_FAIL("bad number of simultaneous transforms");
+  Job j;
+  j.deriv = (type == SHARP_ALM2MAP_DERIV1);
+  if (j.deriv) spin = 1;
+  if (type == SHARP_MAP2ALM) flags |= SHARP_USE_WEIGHTS;
+  if (type == SHARP_Yt) type = SHARP_MAP2ALM;
+  if (type == SHARP_WY) { type = SHARP_ALM2MAP; flags |= SHARP_USE_WEIGHTS; }
+  if (spin < 0 || spin > a->lmax) SHP_FAIL("bad spin");
+  j.type = (type == SHARP_MAP2ALM) ? SHARP_MAP2ALM : SHARP_ALM2MAP;
+  j.spin = spin;
+  j.nmaps = (j.deriv || spin > 0) ? 2 : 1;
+  j.nalm = j.deriv ? 1 : ((spin > 0) ? 2 : 1);
+  j.ntrans = ntrans;
+  j.flags = flags & ~SHARP_NVMAX;
+  j.g = g;
+  j.a = a;
+  j.lmax = a->lmax;
+  // every m from 0 to nm-1 exactly once (libsharp/sharp.c:238-251)
+  std::vector<char> seen(a->nm, 0);
+  for (int i = 0; i < a->nm; ++i) {
+    const int mc = a->mval[i];
+    if (mc < 0 || mc >= a->nm) SHP_FAIL("not all m values are present");
+    if (seen[mc]) SHP_FAIL("duplicate m value");
+    seen[mc] = 1;
+  }
+  j.mmax = a->nm - 1;
+  if (j.lmax < spin) SHP_FAIL("incorrect l_max: must be >= spin");
+  if (j.lmax < j.mmax) SHP_FAIL("incorrect l_max: must be >= m_max");
+  if (j.flags & SHARP_NO_FFT) {
+    for (int i = 0; i < g->npairs; ++i) {
+      if (g->pair[i].r1.nph != j.mmax + 1) SHP_FAIL("bad ring size");
+      if (g->pair[i].r2.nph > 0 && g->pair[i].r2.nph != j.mmax + 1)
+        SHP_FAIL("bad ring size");
+    }
+  }
+  if (j.flags & (SHARP_HIP_MIXED_FP32 | SHARP_HIP_MIXED_BF16X3))
+    SHP_FAIL("mixed-precision forms are provided by the batched entry points");
+  return j;
+}
+
+static unsigned long long op_count(const Job &j, const GeomPlan &gp,
+                                   const GeomPlan::Mlim &ml)
+{
+  unsigned long long triples = 0;
+  for (int k = 0; k < gp.nslots; ++k) {
+    const long long M = std::min(ml.h[k], j.mmax);
+    if (M < 0) continue;
+    triples += (unsigned long long)((M + 1) * (long long)(j.lmax + 1) - M * (M + 1) / 2);
+  }
+  const unsigned long long per =
+      j.deriv ? 12 + 8ull * j.ntrans
+              : (j.spin == 0 ? 4 + 4ull * j.ntrans : 12 + 16ull * j.ntrans);
+  return triples * per;
+}
+
+// ---------------------------------------------------------------------------
+// device pipeline
+// ---------------------------------------------------------------------------
+static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
+                       int64_t alm_base, int64_t map_base, hipStream_t st,
+                       unsigned long long *opcnt)
+{
+  GeomPlan &gp = get_geom(c, j.g);
+  GeomPlan::Mlim &ml = get_mlim(gp, j.lmax, j.spin);
+  YlmPlan &yp = get_ylm(c, j.lmax, j.mmax, j.spin);
+  double *d_norm = get_norm(c, j.lmax, j.spin, j.deriv);
+  int64_t *d_mvs = get_almplan(c, j.a, j.mmax);
+  if (opcnt) *opcnt = op_count(j, gp, ml);
+
+  const int ncomp_alm = j.ntrans * j.nalm, ncomp_map = j.ntrans * j.nmaps;
+  const int nmb = (j.mmax + MB) / MB;
+  const size_t nphase = (size_t)ncomp_map * nmb * gp.nslots * 2 * MB;
+  double2 *phase = static_cast<double2 *>(c.phase.get(nphase * sizeof(double2)));
+  const size_t nalmt = (size_t)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm;
+  double2 *almtmp = static_cast<double2 *>(c.almtmp.get(nalmt * sizeof(double2)));
+
+  AlmArgs aa{};
+  aa.lmax = j.lmax;
+  aa.mmax = j.mmax;
+  aa.ncomp = ncomp_alm;
+  aa.flags = j.flags;
+  aa.packed = j.a->flags & SHARP_PACKED;
+  aa.stride = j.a->stride;
+  aa.alm_base = alm_base;
+  aa.mvstart_by_m = d_mvs;
+  aa.norm_l = d_norm;
+  aa.almtmp = almtmp;
+  for (int i = 0; i < ncomp_alm; ++i) aa.alms.p[i] = dalm[i];
+
+  RingArgs ra{};
+  ra.nslots = gp.nslots;
+  ra.nmb = nmb;
+  ra.mmax = j.mmax;
+  ra.ncomp = ncomp_map;
+  ra.flags = j.flags;
+  ra.map_base = map_base;
+  ra.ring = gp.ring;
+  ra.phase = phase;
+  ra.nphmax = gp.nphmax;
+  ra.dp = (j.flags & SHARP_DP) ? 1 : 0;
+  for (int i = 0; i < ncomp_map; ++i) ra.maps.p[i] = dmap[i];
+
+  LegArgs la{};
+  la.lmax = j.lmax;
+  la.mmax = j.mmax;
+  la.nslots = gp.nslots;
+  la.nmb = nmb;
+  la.spin = j.spin;
+  la.deriv1 = j.deriv;
+  la.ncomp_alm = ncomp_alm;
+  la.ncomp_map = ncomp_map;
+  la.cth = gp.cth;
+  la.sth = gp.sth;
+  la.mlim = ml.d;
+  la.ispair = gp.ispair;
+  la.tab = yp.tab;
+  la.mfac = yp.mfac;
+  la.prefac = yp.prefac;
+  la.fscale = yp.fscale;
+  la.alm = almtmp;
+  la.almout = almtmp;
+  la.phase = phase;
+
+  const int gmax = (j.spin == 0) ? 4 : 2;
+  auto for_groups = [&](auto &&fn) {
+    int t = 0;
+    while (t < j.ntrans) {
+      int g = gmax;
+      while (g > j.ntrans - t) g >>= 1;
+      la.t0 = t;
+      fn(g);
+      t += g;
+    }
+  };
+
+  if (j.type == SHARP_ALM2MAP) {
+    launch_alm_gather(aa, st);
+    for_groups([&](int g) { launch_synth(la, g, st); });
+    launch_phase2ring_dft(ra, st);
+  } else {
+    launch_ring2phase_dft(ra, st);
+    for_groups([&](int g) { launch_anal(la, g, st); });
+    launch_alm_scatter(aa, st);
+  }
+}
+
+// extent [lo, hi] (in real units) touched by an a_lm array
+static void alm_extent(const sharp_alm_info *a, int64_t &lo, int64_t &hi)
+{
+  lo = INT64_MAX;
+  hi = INT64_MIN;
+  const bool packed = a->flags & SHARP_PACKED;
+  for (int mi = 0; mi < a->nm; ++mi) {
+    const int m = a->mval[mi];
+    const bool realonly = packed && m == 0;
+    int64_t ofs = a->mvstart[mi], stride = a->stride;
+    if (!packed) ofs *= 2;
+    if (!realonly) stride *= 2;
+    const int64_t e0 = ofs + (int64_t)m * stride, e1 = ofs + (int64_t)a->lmax * stride;
+    const int64_t add = realonly ? 0 : 1;
+    lo = std::min(lo, std::min(e0, e1));
+    hi = std::max(hi, std::max(e0, e1) + add);
+  }
+}
+
+// extent [lo, hi] (in pixel units) touched by a map
+static void map_extent(const sharp_geom_info *g, int64_t &lo, int64_t &hi)
+{
+  lo = INT64_MAX;
+  hi = INT64_MIN;
+  for (int i = 0; i < g->npairs; ++i)
+    for (const sharp_ringinfo *r : {&g->pair[i].r1, &g->pair[i].r2}) {
+      if (r->nph <= 0) continue;
+      const int64_t e0 = r->ofs, e1 = r->ofs + (int64_t)(r->nph - 1) * r->stride;
+      lo = std::min(lo, std::min(e0, e1));
+      hi = std::max(hi, std::max(e0, e1));
+    }
+}
+
+} // namespace shp
+
+using namespace shp;
+
+extern "C" {
+
+void sharp_execute(sharp_jobtype type, int spin, void *alm, void *map,
+                   const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+                   int ntrans, int flags, double *time, unsigned long long *opcnt)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  std::lock_guard<std::mutex> lock(g_mutex);
+  const Job j = make_job(type, spin, geom_info, alm_info, ntrans, flags);
+  Ctx &c = context();
+  const int ncomp_alm = j.ntrans * j.nalm, ncomp_map = j.ntrans * j.nmaps;
+  const size_t esz = (j.flags & SHARP_DP) ? sizeof(double) : sizeof(float);
+
+  int64_t alo, ahi, mlo, mhi;
+  alm_extent(alm_info, alo, ahi);
+  map_extent(geom_info, mlo, mhi);
+  const bool nofft = j.flags & SHARP_NO_FFT;
+  const int64_t mreal_lo = nofft ? 2 * mlo : mlo, mreal_hi = nofft ? 2 * mhi + 1 : mhi;
+  const size_t an = (size_t)(ahi - alo + 1), mn = (size_t)(mreal_hi - mreal_lo + 1);
+  char *da = static_cast<char *>(c.stage_alm.get(an * esz * ncomp_alm));
+  char *dm = static_cast<char *>(c.stage_map.get(mn * esz * ncomp_map));
+  void **halm = static_cast<void **>(alm), **hmap = static_cast<void **>(map);
+  std::vector<void *> dalm(ncomp_alm), dmap(ncomp_map);
+  hipStream_t st = c.stream;
+  for (int i = 0; i < ncomp_alm; ++i) {
+    dalm[i] = da + i * an * esz;
+    HIP_CHECK(hipMemcpyAsync(dalm[i], static_cast<char *>(halm[i]) + alo * (int64_t)esz,
+                             an * esz, hipMemcpyHostToDevice, st));
+  }
+  for (int i = 0; i < ncomp_map; ++i) {
+    dmap[i] = dm + i * mn * esz;
+    HIP_CHECK(hipMemcpyAsync(dmap[i], static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz,
+                             mn * esz, hipMemcpyHostToDevice, st));
+  }
+  run_device(c, j, dalm.data(), dmap.data(), alo, mlo, st, opcnt);
+  if (j.type == SHARP_ALM2MAP) {
+    for (int i = 0; i < ncomp_map; ++i)
+      HIP_CHECK(hipMemcpyAsync(static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz,
+                               dmap[i], mn * esz, hipMemcpyDeviceToHost, st));
+  } else {
+    for (int i = 0; i < ncomp_alm; ++i)
+      HIP_CHECK(hipMemcpyAsync(static_cast<char *>(halm[i]) + alo * (int64_t)esz,
+                               dalm[i], an * esz, hipMemcpyDeviceToHost, st));
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (time)
+    *time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void sharp_hip_execute_device(sharp_jobtype type, int spin, void *alm, void *map,
+                              const sharp_geom_info *geom_info,
+                              const sharp_alm_info *alm_info, int ntrans, int flags,
+                              void *stream, unsigned long long *opcnt)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  const Job j = make_job(type, spin, geom_info, alm_info, ntrans, flags);
+  Ctx &c = context();
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  run_device(c, j, static_cast<void *const *>(alm), static_cast<void *const *>(map),
+             0, 0, st, opcnt);
+}
+
+void sharp_hip_alm2map_adjoint_device(int spin, void *alm, void *map,
+                                      const sharp_geom_info *geom_info,
+                                      const sharp_alm_info *alm_info, int ntrans,
+                                      int flags, void *stream)
+{
+  sharp_hip_execute_device(SHARP_Yt, spin, alm, map, geom_info, alm_info, ntrans,
+                           flags, stream, nullptr);
+}
+
+void sharp_hip_map2alm_adjoint_device(int spin, void *alm, void *map,
+                                      const sharp_geom_info *geom_info,
+                                      const sharp_alm_info *alm_info, int ntrans,
+                                      int flags, void *stream)
+{
+  sharp_hip_execute_device(SHARP_WY, spin, alm, map, geom_info, alm_info, ntrans,
+                           flags, stream, nullptr);
+}
+
+void sharp_hip_prepare(const sharp_geom_info *geom_info,
+                       const sharp_alm_info *alm_info, int spin)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Ctx &c = context();
+  const int mmax = alm_info->nm - 1;
+  GeomPlan &gp = get_geom(c, geom_info);
+  get_mlim(gp, alm_info->lmax, spin);
+  get_ylm(c, alm_info->lmax, mmax, spin);
+  get_norm(c, alm_info->lmax, spin, 0);
+  get_almplan(c, alm_info, mmax);
+}
+
+void sharp_hip_synchronize(void *stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Ctx &c = context();
+  HIP_CHECK(hipStreamSynchronize(stream ? static_cast<hipStream_t>(stream) : c.stream));
+}
+
+void sharp_hip_release(void)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  auto it = g_ctx.find(dev);
+  if (it == g_ctx.end()) return;
+  Ctx &c = *it->second;
+  (void)hipDeviceSynchronize();
+  c.geoms.clear();
+  c.ylms.clear();
+  c.norms.clear();
+  c.alms.clear();
+  c.phase.release();
+  c.almtmp.release();
+  c.stage_alm.release();
+  c.stage_map.release();
+}
+
+int sharp_hip_device_count(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+} // extern "C"

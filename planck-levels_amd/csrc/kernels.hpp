@@ -1,0 +1,69 @@
+// kernels.hpp -- launch interface between the host engine and the HIP
+// kernels.  Plain structs of device pointers; no torch types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "shp_internal.hpp"
+
+namespace shp {
+
+// Arguments of the Legendre (m <-> ring pair) stage.
+struct LegArgs {
+  int lmax, mmax, nslots, nmb;
+  int spin;             // 0, or s>0 for the Wigner-d recurrences
+  int deriv1;           // 1: spin-1 synthesis of a gradient (no curl a_lm)
+  int ncomp_alm;        // components in almtmp   (ntrans*nalm)
+  int ncomp_map;        // components in phase    (ntrans*nmaps)
+  int t0;               // first transform handled by this launch
+  const double *cth, *sth;
+  const int *mlim, *ispair;
+  const double *tab;    // recurrence coefficients, rows at tab_ofs(m,lmax)
+  const double *mfac;   // spin 0: lambda_mm prefactors [mmax+1]
+  const double *prefac; // spin s: start-value prefactors [mmax+1]
+  const int *fscale;    // spin s: their scales [mmax+1]
+  const double2 *alm;   // synthesis input  (canonical almtmp)
+  double2 *almout;      // analysis output  (canonical almtmp)
+  double2 *phase;
+};
+
+// ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
+void launch_synth(const LegArgs &a, int ntg, hipStream_t st);
+void launch_anal(const LegArgs &a, int ntg, hipStream_t st);
+
+// Arguments of the ring stage (phase <-> map).
+constexpr int kMaxComp = 200;   // ntrans <= 100, two components each
+
+// Per-component device pointers, passed by value in the kernel arguments.
+struct PtrTab {
+  void *p[kMaxComp];
+};
+
+struct RingArgs {
+  int nslots, nmb, mmax, ncomp;
+  int flags;            // SHARP_* flags after job normalisation
+  int64_t map_base;     // subtracted from every pixel index
+  const RingDev *ring;  // [nslots][2]
+  double2 *phase;
+  int nphmax;
+  int dp;               // 1: double maps, 0: float maps
+  PtrTab maps;          // [ncomp] device pointers (float or double)
+};
+void launch_phase2ring_dft(const RingArgs &a, hipStream_t st);
+void launch_ring2phase_dft(const RingArgs &a, hipStream_t st);
+
+// a_lm in the caller's layout <-> canonical almtmp.
+struct AlmArgs {
+  int lmax, mmax, ncomp;
+  int flags;            // job flags (ADD, REAL_HARMONICS, DP)
+  int packed;           // alm_info flags & SHARP_PACKED
+  int64_t stride;       // alm_info stride
+  int64_t alm_base;     // subtracted from every a_lm index (real units)
+  const int64_t *mvstart_by_m;  // [mmax+1]
+  const double *norm_l; // [lmax+1]
+  double2 *almtmp;
+  PtrTab alms;          // [ncomp] device pointers (float or double data)
+};
+void launch_alm_gather(const AlmArgs &a, hipStream_t st);
+void launch_alm_scatter(const AlmArgs &a, hipStream_t st);
+
+} // namespace shp

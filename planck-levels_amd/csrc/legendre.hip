@@ -1,0 +1,784 @@
+// legendre.hip -- FP64 Legendre stage of the SHT on CDNA4 (gfx950).
+//
+// Synthesis (a_lm -> phase): one wavefront per (m, 64 ring pairs).  Each lane
+// owns one ring pair and runs the three-term recurrence over l in registers;
+// the per-m recurrence coefficients and the a_lm of that m are wave-uniform
+// and are fetched through the scalar data path (s_load), so the vector ALUs
+// only see the recurrence and the accumulation.  A workgroup is MB=8
+// wavefronts with consecutive m over the same 64 ring pairs, which fills the
+// 128-byte phase blocks phase[comp][m/8][slot][hemi][0..7].
+//
+// Analysis (phase -> a_lm): one workgroup (4 wavefronts) per m, looping over
+// the ring pairs 256 at a time.  Each lane accumulates lambda_lm(ring) *
+// phase(ring) for 8 consecutive l, the wavefront reduces the 8 l-values
+// across its 64 lanes with a transpose-reduce (each exchange halves the
+// number of values a lane carries), the 4 wavefronts combine in LDS and one
+// fixed thread per output adds into the canonical a_lm row of this m, so the
+// summation order is deterministic.
+//
+// Exponent scaling follows libsharp: start values sin^m(theta) are formed
+// with a separate power-of-2 exponent, the recurrence runs in the scaled
+// domain until some lane reaches the IEEE range, then with per-lane
+// correction factors until every lane has, then unscaled.  "Some lane" /
+// "every lane" are wavefront ballots here where libsharp uses its SIMD
+// block (libsharp/sharp_core_inc.c:146-207, sharp_core_inc2.c:164-280).
+//
+// Spin s > 0 uses the pair of Wigner-d recurrences lambda_plus / lambda_minus
+// (libsharp/sharp_core_inc.c:209-293, sharp_ylmgen_c.c:162-197) and the E/B ->
+// Q/U combination of libsharp/sharp_core_inc2.c:282-430; ALM2MAP_DERIV1 is the
+// same machinery with the curl a_lm identically zero
+// (libsharp/sharp_core_inc2.c:521-619).
+
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+
+namespace shp {
+
+// Brings |val| into [maxval*2^-800, maxval].  A finite double needs at most
+// three 800-binade steps; the bound also ends the loop for non-finite input.
+__device__ inline void normalize(double &val, int &scale, double maxval)
+{
+  for (int it = 0; it < 4 && fabs(val) > maxval; ++it) { val *= kFSmall; ++scale; }
+  if (val != 0.)
+    for (int it = 0; it < 4 && fabs(val) < maxval * kFSmall; ++it) {
+      val *= kFBig;
+      --scale;
+    }
+}
+
+// val^npow with a separate exponent (libsharp/sharp_core_inc.c:97-126)
+__device__ inline void mypow(double val, int npow, double &resd, int &ress)
+{
+  int scale = 0, scaleint = 0;
+  double res = 1.;
+  normalize(val, scaleint, kFBigHalf);
+  do {
+    if (npow & 1) {
+      res *= val;
+      scale += scaleint;
+      normalize(res, scale, kFBigHalf);
+    }
+    val *= val;
+    scaleint += scaleint;
+    normalize(val, scaleint, kFBigHalf);
+  } while (npow >>= 1);
+  resd = res;
+  ress = scale;
+}
+
+__device__ inline double corfac(int scale)
+{
+  return (scale < 0) ? 0. : ((scale == 0) ? 1. : kFBig);
+}
+
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
+__device__ inline bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+
+// ---------------------------------------------------------------------------
+// transpose-reduce over the 64 lanes of a wavefront.  On entry every lane
+// holds C values; on exit v[0] of lane L holds the sum over all lanes of value
+// number xred_index<C>(L), replicated over the 64/C lanes that differ only in
+// their low log2(64/C) bits.
+// ---------------------------------------------------------------------------
+template <int C, int D> struct XRed {
+  static __device__ inline void run(double *v, int lane)
+  {
+    if constexpr (D == 0) {
+      return;
+    } else if constexpr (C > 1) {
+      const bool up = (lane & D) != 0;
+#pragma unroll
+      for (int i = 0; i < C / 2; ++i) {
+        const double send = up ? v[i] : v[i + C / 2];
+        const double keep = up ? v[i + C / 2] : v[i];
+        v[i] = keep + __shfl_xor(send, D);
+      }
+      XRed<C / 2, D / 2>::run(v, lane);
+    } else {
+      v[0] += __shfl_xor(v[0], D);
+      XRed<1, D / 2>::run(v, lane);
+    }
+  }
+};
+
+template <int C> __device__ inline int xred_index(int lane)
+{
+  int idx = 0, c = C;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    if (c > 1) {
+      if (lane & d) idx += c / 2;
+      c /= 2;
+    }
+  }
+  return idx;
+}
+
+// ===========================================================================
+// spin 0 synthesis
+// ===========================================================================
+template <int TG>
+__global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a)
+{
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int m = blockIdx.y * MB + w;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax;
+  const int slot = blockIdx.x * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+
+  double p1r[TG], p1i[TG], p2r[TG], p2i[TG];
+#pragma unroll
+  for (int t = 0; t < TG; ++t) p1r[t] = p1i[t] = p2r[t] = p2i[t] = 0.;
+
+  if (wave_any(active)) {
+    const double2 *rf =
+        reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+    const int nc = a.ncomp_alm;
+    const double2 *al = a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0);
+
+    double lam1 = 0., lam2;
+    int scale;
+    mypow(sth, m, lam2, scale);
+    const double mf = a.mfac[m];
+    lam2 *= (m & 1) ? -mf : mf;
+    normalize(lam2, scale, kFTol);
+
+    int l = m;
+    bool dead = false;
+    // below the IEEE range on every lane: advance without accumulating
+    while (wave_all(scale < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      const double2 r0 = rf[l], r1 = rf[l + 1];
+      lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+      lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+      if (fabs(lam2) > kFTol) { lam1 *= kFSmall; lam2 *= kFSmall; ++scale; }
+      l += 2;
+    }
+    if (!dead) {
+      double cf = corfac(scale);
+      bool done = false;
+      // some lanes still scaled: accumulate with correction factors
+      while (wave_any(scale < 0)) {
+        {
+          const double tmp = lam2 * cf;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            const double2 x = al[(int64_t)l * nc + t];
+            p1r[t] += tmp * x.x; p1i[t] += tmp * x.y;
+          }
+        }
+        if (++l > lmax) { done = true; break; }
+        {
+          const double2 r = rf[l - 1];
+          lam1 = (cth * lam2) * r.x - lam1 * r.y;
+          const double tmp = lam1 * cf;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            const double2 x = al[(int64_t)l * nc + t];
+            p2r[t] += tmp * x.x; p2i[t] += tmp * x.y;
+          }
+        }
+        if (++l > lmax) { done = true; break; }
+        const double2 r = rf[l - 1];
+        lam2 = (cth * lam1) * r.x - lam2 * r.y;
+        if (fabs(lam2) > kFTol) {
+          lam1 *= kFSmall; lam2 *= kFSmall; ++scale;
+          cf = corfac(scale);
+        }
+      }
+      if (!done) {
+        lam1 *= cf;
+        lam2 *= cf;
+        // hot loop: 3 FP64 ops of recurrence + 4*TG of accumulation per l pair/2
+        while (l < lmax) {
+          const double2 r0 = rf[l], r1 = rf[l + 1];
+          lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            const double2 x0 = al[(int64_t)l * nc + t];
+            const double2 x1 = al[(int64_t)(l + 1) * nc + t];
+            p1r[t] = fma(lam2, x0.x, p1r[t]); p1i[t] = fma(lam2, x0.y, p1i[t]);
+            p2r[t] = fma(lam1, x1.x, p2r[t]); p2i[t] = fma(lam1, x1.y, p2i[t]);
+          }
+          lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+          l += 2;
+        }
+        if (l == lmax) {
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            const double2 x = al[(int64_t)l * nc + t];
+            p1r[t] += lam2 * x.x; p1i[t] += lam2 * x.y;
+          }
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  const bool pr = a.ispair[slot] != 0;
+#pragma unroll
+  for (int t = 0; t < TG; ++t) {
+    double2 n = make_double2(0., 0.), sv = make_double2(0., 0.);
+    if (active) {
+      n = make_double2(p1r[t] + p2r[t], p1i[t] + p2i[t]);
+      sv = make_double2(p1r[t] - p2r[t], p1i[t] - p2i[t]);
+    }
+    a.phase[phase_index(a.t0 + t, m, slot, 0, a.nslots, a.nmb)] = n;
+    if (pr) a.phase[phase_index(a.t0 + t, m, slot, 1, a.nslots, a.nmb)] = sv;
+  }
+}
+
+// ===========================================================================
+// spin s synthesis (and ALM2MAP_DERIV1)
+// ===========================================================================
+struct SpinStart {
+  int mhi, cosPow, sinPow, preMinus_p, preMinus_m;
+};
+
+__device__ inline SpinStart spin_start(int m, int s)
+{
+  SpinStart r;
+  const int mhi = (m < s) ? s : m;
+  r.mhi = mhi;
+  r.preMinus_p = r.preMinus_m = 0;
+  if (mhi == m) {
+    r.cosPow = mhi + s;
+    r.sinPow = mhi - s;
+    r.preMinus_p = r.preMinus_m = ((mhi - s) & 1);
+  } else {
+    r.cosPow = mhi + m;
+    r.sinPow = mhi - m;
+    r.preMinus_m = ((mhi + m) & 1);
+  }
+  return r;
+}
+
+// start values of lambda_plus / lambda_minus at l=mhi with their scales
+// (libsharp/sharp_core_inc.c:223-273)
+__device__ inline void spin_init(double cth, double sth, int s, const SpinStart &st,
+                                 double prefac, int fscale, double &rec2p,
+                                 double &rec2m, int &scalep, int &scalem)
+{
+  double cth2 = fmax(sqrt((1. + cth) * 0.5), 1e-15);
+  double sth2 = fmax(sqrt((1. - cth) * 0.5), 1e-15);
+  if (sth < 0. && cth < 0.) cth2 = -cth2;
+  if (sth < 0. && cth > 0.) sth2 = -sth2;
+  double ccp, ssp, csp, scp;
+  int ccps, ssps, csps, scps;
+  mypow(cth2, st.cosPow, ccp, ccps);
+  mypow(sth2, st.sinPow, ssp, ssps);
+  mypow(cth2, st.sinPow, csp, csps);
+  mypow(sth2, st.cosPow, scp, scps);
+  rec2p = prefac * ccp; scalep = fscale + ccps;
+  rec2m = prefac * csp; scalem = fscale + csps;
+  normalize(rec2m, scalem, kFBigHalf);
+  normalize(rec2p, scalep, kFBigHalf);
+  rec2p *= ssp; scalep += ssps;
+  rec2m *= scp; scalem += scps;
+  if (st.preMinus_p) rec2p = -rec2p;
+  if (st.preMinus_m) rec2m = -rec2m;
+  if (s & 1) rec2p = -rec2p;
+  normalize(rec2m, scalem, kFTol);
+  normalize(rec2p, scalep, kFTol);
+}
+
+struct Fx { double f0, f1, f2; };
+
+__device__ inline void rec_step(double &rxp, double &rxm, double ryp, double rym,
+                                double cth, const Fx &f)
+{
+  rxp = ((cth - f.f1) * (f.f0 * ryp)) - f.f2 * rxp;
+  rxm = ((cth + f.f1) * (f.f0 * rym)) - f.f2 * rxm;
+}
+
+__device__ inline bool rescale1(double &l1, double &l2, int &scale)
+{
+  if (fabs(l2) > kFTol) {
+    l1 *= kFSmall; l2 *= kFSmall; ++scale;
+    return true;
+  }
+  return false;
+}
+
+struct QU { double qr, qi, ur, ui; };
+
+// px += lw*(E,B), py += lx*rot(E,B)  (libsharp/sharp_core_inc2.c:282-306)
+template <bool DERIV>
+__device__ inline void saddstep(QU &px, QU &py, double rxp, double rxm,
+                                double2 ag, double2 ac)
+{
+  const double lw = rxp + rxm, lx = rxm - rxp;
+  px.qr = fma(ag.x, lw, px.qr);
+  px.qi = fma(ag.y, lw, px.qi);
+  if (!DERIV) {
+    px.ur = fma(ac.x, lw, px.ur);
+    px.ui = fma(ac.y, lw, px.ui);
+    py.qr = fma(-ac.y, lx, py.qr);
+    py.qi = fma(ac.x, lx, py.qi);
+  }
+  py.ur = fma(ag.y, lx, py.ur);
+  py.ui = fma(-ag.x, lx, py.ui);
+}
+
+template <int TG, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a)
+{
+  constexpr int NALM = DERIV ? 1 : 2;
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int m = blockIdx.y * MB + w;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax, sp = a.spin;
+  const int slot = blockIdx.x * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+  const SpinStart st = spin_start(m, sp);
+
+  QU p1[TG], p2[TG];
+#pragma unroll
+  for (int t = 0; t < TG; ++t) {
+    p1[t] = QU{0., 0., 0., 0.};
+    p2[t] = QU{0., 0., 0., 0.};
+  }
+
+  if (wave_any(active) && st.mhi <= lmax) {
+    const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+    auto fx = [&](int l) { return Fx{fxb[3 * l], fxb[3 * l + 1], fxb[3 * l + 2]}; };
+    const int nc = a.ncomp_alm;
+    const double2 *al = a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0 * NALM);
+    auto ag = [&](int l, int t) { return al[(int64_t)l * nc + t * NALM]; };
+    auto ac = [&](int l, int t) {
+      return DERIV ? make_double2(0., 0.) : al[(int64_t)l * nc + t * NALM + 1];
+    };
+
+    double rec1p = 0., rec1m = 0., rec2p, rec2m;
+    int scalep, scalem;
+    spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep,
+              scalem);
+    int l = st.mhi;
+    bool dead = false;
+    while (wave_all(scalep < 1 && scalem < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+      rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+      rescale1(rec1p, rec2p, scalep);
+      rescale1(rec1m, rec2m, scalem);
+      l += 2;
+    }
+    if (!dead) {
+      double cfp = corfac(scalep), cfm = corfac(scalem);
+      bool done = false;
+      while (wave_any(scalep < 0 || scalem < 0)) {
+#pragma unroll
+        for (int t = 0; t < TG; ++t)
+          saddstep<DERIV>(p1[t], p2[t], rec2p * cfp, rec2m * cfm, ag(l, t), ac(l, t));
+        if (++l > lmax) { done = true; break; }
+        rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l));
+#pragma unroll
+        for (int t = 0; t < TG; ++t)
+          saddstep<DERIV>(p2[t], p1[t], rec1p * cfp, rec1m * cfm, ag(l, t), ac(l, t));
+        if (++l > lmax) { done = true; break; }
+        rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l));
+        const bool a1 = rescale1(rec1p, rec2p, scalep);
+        const bool a2 = rescale1(rec1m, rec2m, scalem);
+        if (a1 || a2) { cfp = corfac(scalep); cfm = corfac(scalem); }
+      }
+      if (!done) {
+        rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+        while (l < lmax) {
+          rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, ag(l, t), ac(l, t));
+            saddstep<DERIV>(p2[t], p1[t], rec1p, rec1m, ag(l + 1, t), ac(l + 1, t));
+          }
+          rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+          l += 2;
+        }
+        if (l == lmax) {
+#pragma unroll
+          for (int t = 0; t < TG; ++t)
+            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, ag(l, t), ac(l, t));
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  const bool pr = a.ispair[slot] != 0;
+  const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
+#pragma unroll
+  for (int t = 0; t < TG; ++t) {
+    const int cq = 2 * (a.t0 + t), cu = cq + 1;
+    double2 qn = make_double2(0., 0.), un = qn, qs = qn, us = qn;
+    if (active) {
+      qn = make_double2(p1[t].qr + p2[t].qr, p1[t].qi + p2[t].qi);
+      un = make_double2(p1[t].ur + p2[t].ur, p1[t].ui + p2[t].ui);
+      qs = make_double2(sg * (p1[t].qr - p2[t].qr), sg * (p1[t].qi - p2[t].qi));
+      us = make_double2(sg * (p1[t].ur - p2[t].ur), sg * (p1[t].ui - p2[t].ui));
+    }
+    a.phase[phase_index(cq, m, slot, 0, a.nslots, a.nmb)] = qn;
+    a.phase[phase_index(cu, m, slot, 0, a.nslots, a.nmb)] = un;
+    if (pr) {
+      a.phase[phase_index(cq, m, slot, 1, a.nslots, a.nmb)] = qs;
+      a.phase[phase_index(cu, m, slot, 1, a.nslots, a.nmb)] = us;
+    }
+  }
+}
+
+// ===========================================================================
+// spin 0 analysis
+// ===========================================================================
+constexpr int kChunk = 8;   // l values per reduction chunk
+
+template <int TG>
+__global__ void __launch_bounds__(256) k_anal_spin0(LegArgs a)
+{
+  constexpr int K = kChunk;
+  constexpr int NV = 2 * K * TG;          // doubles per lane per chunk
+  constexpr int BMASK = 64 / NV - 1;      // low lane bits of replicated results
+  static_assert(NV <= 64, "too many values per chunk");
+  __shared__ double red[2][4][NV];
+
+  const int m = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lmax = a.lmax, nc = a.ncomp_alm;
+  const int nchunks = (lmax - m + K) / K;
+  const int ngroups = (a.nslots + 255) / 256;
+  double *out = reinterpret_cast<double *>(a.almout + ((tri_ofs(m, lmax) - m) * nc + a.t0));
+  const double2 *rf = reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+
+  // zero the output row of this m (all components of this launch)
+  for (int i = threadIdx.x; i < (lmax - m + 1) * TG * 2; i += 256) {
+    const int l = m + i / (2 * TG), r = i % (2 * TG);
+    out[(int64_t)l * nc * 2 + r] = 0.;
+  }
+  __syncthreads();
+
+  const bool writer = (w == 0) && (lane < NV);
+  const int wt = lane / (2 * K), wj = lane % (2 * K);
+  const int myidx = xred_index<NV>(lane);
+  const bool holder = (lane & BMASK) == 0;
+  int bufsel = 0;
+
+  for (int g = 0; g < ngroups; ++g) {
+    const int slot = g * 256 + w * 64 + lane;
+    const bool valid = slot < a.nslots;
+    const int s = valid ? slot : a.nslots - 1;
+    const bool active = valid && (a.mlim[s] >= m);
+    if (!__syncthreads_or(active)) continue;
+    const double cth = a.cth[s], sth = a.sth[s];
+    double p1r[TG], p1i[TG], p2r[TG], p2i[TG];
+    const bool pr = a.ispair[s] != 0;
+#pragma unroll
+    for (int t = 0; t < TG; ++t) {
+      p1r[t] = p1i[t] = p2r[t] = p2i[t] = 0.;
+      if (active) {
+        const double2 n = a.phase[phase_index(a.t0 + t, m, slot, 0, a.nslots, a.nmb)];
+        double2 sv = make_double2(0., 0.);
+        if (pr) sv = a.phase[phase_index(a.t0 + t, m, slot, 1, a.nslots, a.nmb)];
+        p1r[t] = n.x + sv.x; p1i[t] = n.y + sv.y;
+        p2r[t] = n.x - sv.x; p2i[t] = n.y - sv.y;
+      }
+    }
+    double lam1 = 0., lam2;
+    int scale;
+    mypow(sth, m, lam2, scale);
+    const double mf = a.mfac[m];
+    lam2 *= (m & 1) ? -mf : mf;
+    normalize(lam2, scale, kFTol);
+    // 0: advancing below the IEEE range, 1: scaled, 2: unscaled, 3: idle
+    int mode = wave_any(active) ? 0 : 3;
+    double cf = 0.;
+    int l = m;
+
+    for (int c = 0; c < nchunks; ++c) {
+      double acc[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) acc[i] = 0.;
+      bool contrib = false;
+      if (mode == 2) {
+        contrib = true;
+#pragma unroll
+        for (int k = 0; k < K; k += 2) {
+          const double2 r0 = rf[l], r1 = rf[l + 1];
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            acc[t * 2 * K + 2 * k] = lam2 * p1r[t];
+            acc[t * 2 * K + 2 * k + 1] = lam2 * p1i[t];
+          }
+          lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            acc[t * 2 * K + 2 * k + 2] = lam1 * p2r[t];
+            acc[t * 2 * K + 2 * k + 3] = lam1 * p2i[t];
+          }
+          lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+          l += 2;
+        }
+      } else if (mode < 2) {
+#pragma unroll
+        for (int k = 0; k < K; k += 2) {
+          const double2 r0 = rf[l], r1 = rf[l + 1];
+          if (mode == 0) {
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (fabs(lam2) > kFTol) { lam1 *= kFSmall; lam2 *= kFSmall; ++scale; }
+            if (wave_any(scale >= 1)) { mode = 1; cf = corfac(scale); }
+          } else {
+            contrib = true;
+            const double t2 = lam2 * cf;
+#pragma unroll
+            for (int t = 0; t < TG; ++t) {
+              acc[t * 2 * K + 2 * k] = t2 * p1r[t];
+              acc[t * 2 * K + 2 * k + 1] = t2 * p1i[t];
+            }
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            const double t1 = lam1 * cf;
+#pragma unroll
+            for (int t = 0; t < TG; ++t) {
+              acc[t * 2 * K + 2 * k + 2] = t1 * p2r[t];
+              acc[t * 2 * K + 2 * k + 3] = t1 * p2i[t];
+            }
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (fabs(lam2) > kFTol) {
+              lam1 *= kFSmall; lam2 *= kFSmall; ++scale;
+              cf = corfac(scale);
+            }
+          }
+          l += 2;
+        }
+        if (mode == 1 && wave_all(scale >= 0)) {
+          lam1 *= cf; lam2 *= cf;
+          mode = 2;
+        }
+      }
+      if (contrib) XRed<NV, 32>::run(acc, lane);
+      if (holder) red[bufsel][w][myidx] = contrib ? acc[0] : 0.;
+      __syncthreads();
+      if (writer) {
+        const double v = red[bufsel][0][lane] + red[bufsel][1][lane] +
+                         red[bufsel][2][lane] + red[bufsel][3][lane];
+        const int ll = m + K * c + wj / 2;
+        if (ll <= lmax) out[((int64_t)ll * nc + wt) * 2 + (wj & 1)] += v;
+      }
+      bufsel ^= 1;
+    }
+  }
+}
+
+// ===========================================================================
+// spin s analysis
+// ===========================================================================
+template <int TG>
+__global__ void __launch_bounds__(256) k_anal_spin(LegArgs a)
+{
+  constexpr int K = kChunk;
+  constexpr int NV = 4 * K * TG;
+  constexpr int BMASK = 64 / NV - 1;
+  static_assert(NV <= 64, "too many values per chunk");
+  __shared__ double red[2][4][NV];
+
+  const int m = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lmax = a.lmax, nc = a.ncomp_alm, sp = a.spin;
+  const SpinStart st = spin_start(m, sp);
+  const int nchunks = (st.mhi <= lmax) ? (lmax - st.mhi + K) / K : 0;
+  const int ngroups = (a.nslots + 255) / 256;
+  double *out = reinterpret_cast<double *>(a.almout + ((tri_ofs(m, lmax) - m) * nc + 2 * a.t0));
+  const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+  auto fx = [&](int l) { return Fx{fxb[3 * l], fxb[3 * l + 1], fxb[3 * l + 2]}; };
+
+  for (int i = threadIdx.x; i < (lmax - m + 1) * TG * 4; i += 256) {
+    const int l = m + i / (4 * TG), r = i % (4 * TG);
+    out[(int64_t)l * nc * 2 + r] = 0.;
+  }
+  __syncthreads();
+
+  const bool writer = (w == 0) && (lane < NV);
+  const int wt = lane / (4 * K), wj = lane % (4 * K);
+  const int myidx = xred_index<NV>(lane);
+  const bool holder = (lane & BMASK) == 0;
+  const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
+  int bufsel = 0;
+
+  for (int g = 0; g < ngroups && nchunks > 0; ++g) {
+    const int slot = g * 256 + w * 64 + lane;
+    const bool valid = slot < a.nslots;
+    const int s = valid ? slot : a.nslots - 1;
+    const bool active = valid && (a.mlim[s] >= m);
+    if (!__syncthreads_or(active)) continue;
+    const double cth = a.cth[s], sth = a.sth[s];
+    const bool pr = a.ispair[s] != 0;
+    // P[0] = N+S (used for l-mhi even), P[1] = N-S
+    QU P[2][TG];
+#pragma unroll
+    for (int t = 0; t < TG; ++t) {
+      P[0][t] = QU{0., 0., 0., 0.};
+      P[1][t] = QU{0., 0., 0., 0.};
+      if (active) {
+        const int cq = 2 * (a.t0 + t), cu = cq + 1;
+        const double2 nq = a.phase[phase_index(cq, m, slot, 0, a.nslots, a.nmb)];
+        const double2 nu = a.phase[phase_index(cu, m, slot, 0, a.nslots, a.nmb)];
+        double2 sq = make_double2(0., 0.), su = sq;
+        if (pr) {
+          sq = a.phase[phase_index(cq, m, slot, 1, a.nslots, a.nmb)];
+          su = a.phase[phase_index(cu, m, slot, 1, a.nslots, a.nmb)];
+          sq.x *= sg; sq.y *= sg; su.x *= sg; su.y *= sg;
+        }
+        P[0][t] = QU{nq.x + sq.x, nq.y + sq.y, nu.x + su.x, nu.y + su.y};
+        P[1][t] = QU{nq.x - sq.x, nq.y - sq.y, nu.x - su.x, nu.y - su.y};
+      }
+    }
+    double rec1p = 0., rec1m = 0., rec2p, rec2m;
+    int scalep, scalem;
+    spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep,
+              scalem);
+    int mode = wave_any(active) ? 0 : 3;
+    double cfp = 0., cfm = 0.;
+    int l = st.mhi;
+
+    // a(E,B) += saddstep2(px, py, rp, rm) (libsharp/sharp_core_inc2.c:339-364)
+    auto accum = [&](double *acc, int k, int x, double rp, double rm) {
+      const double lw = rp + rm, lx = rm - rp;
+#pragma unroll
+      for (int t = 0; t < TG; ++t) {
+        const QU &px = P[x][t], &py = P[x ^ 1][t];
+        double *o = acc + t * 4 * K + 4 * k;
+        o[0] = px.qr * lw - py.ui * lx;
+        o[1] = px.qi * lw + py.ur * lx;
+        o[2] = px.ur * lw + py.qi * lx;
+        o[3] = px.ui * lw - py.qr * lx;
+      }
+    };
+
+    for (int c = 0; c < nchunks; ++c) {
+      double acc[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) acc[i] = 0.;
+      bool contrib = false;
+      if (mode == 2) {
+        contrib = true;
+#pragma unroll
+        for (int k = 0; k < K; k += 2) {
+          accum(acc, k, 0, rec2p, rec2m);
+          rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+          accum(acc, k + 1, 1, rec1p, rec1m);
+          rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+          l += 2;
+        }
+      } else if (mode < 2) {
+#pragma unroll
+        for (int k = 0; k < K; k += 2) {
+          if (mode == 0) {
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+            rescale1(rec1p, rec2p, scalep);
+            rescale1(rec1m, rec2m, scalem);
+            if (wave_any(scalep >= 1 || scalem >= 1)) {
+              mode = 1;
+              cfp = corfac(scalep); cfm = corfac(scalem);
+            }
+          } else {
+            contrib = true;
+            accum(acc, k, 0, rec2p * cfp, rec2m * cfm);
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+            accum(acc, k + 1, 1, rec1p * cfp, rec1m * cfm);
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+            const bool a1 = rescale1(rec1p, rec2p, scalep);
+            const bool a2 = rescale1(rec1m, rec2m, scalem);
+            if (a1 || a2) { cfp = corfac(scalep); cfm = corfac(scalem); }
+          }
+          l += 2;
+        }
+        if (mode == 1 && wave_all(scalep >= 0 && scalem >= 0)) {
+          rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+          mode = 2;
+        }
+      }
+      if (contrib) XRed<NV, 32>::run(acc, lane);
+      if (holder) red[bufsel][w][myidx] = contrib ? acc[0] : 0.;
+      __syncthreads();
+      if (writer) {
+        const double v = red[bufsel][0][lane] + red[bufsel][1][lane] +
+                         red[bufsel][2][lane] + red[bufsel][3][lane];
+        const int ll = st.mhi + K * c + wj / 4;
+        const int q = wj % 4;   // 0: E re, 1: E im, 2: B re, 3: B im
+        if (ll <= lmax)
+          out[((int64_t)ll * nc + 2 * wt + (q >> 1)) * 2 + (q & 1)] += v;
+      }
+      bufsel ^= 1;
+    }
+  }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static void check_launch(const char *what)
+{
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "kernel launch failed (%s): %s", what,
+             hipGetErrorString(e));
+    SHP_FAIL(buf);
+  }
+}
+
+void launch_synth(const LegArgs &a, int ntg, hipStream_t st)
+{
+  const dim3 grid((a.nslots + 63) / 64, (a.mmax + MB) / MB);
+  const dim3 block(64, MB);
+  if (a.spin == 0) {
+    switch (ntg) {
+    case 1: hipLaunchKernelGGL(k_synth_spin0<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_synth_spin0<2>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_synth_spin0<4>, grid, block, 0, st, a); break;
+    default: SHP_FAIL("bad transform group");
+    }
+  } else if (a.deriv1) {
+    switch (ntg) {
+    case 1: hipLaunchKernelGGL((k_synth_spin<1, true>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_synth_spin<2, true>), grid, block, 0, st, a); break;
+    default: SHP_FAIL("bad transform group");
+    }
+  } else {
+    switch (ntg) {
+    case 1: hipLaunchKernelGGL((k_synth_spin<1, false>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_synth_spin<2, false>), grid, block, 0, st, a); break;
+    default: SHP_FAIL("bad transform group");
+    }
+  }
+  check_launch("synthesis");
+}
+
+void launch_anal(const LegArgs &a, int ntg, hipStream_t st)
+{
+  const dim3 grid(a.mmax + 1);
+  const dim3 block(256);
+  if (a.spin == 0) {
+    switch (ntg) {
+    case 1: hipLaunchKernelGGL(k_anal_spin0<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_anal_spin0<2>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_anal_spin0<4>, grid, block, 0, st, a); break;
+    default: SHP_FAIL("bad transform group");
+    }
+  } else {
+    switch (ntg) {
+    case 1: hipLaunchKernelGGL(k_anal_spin<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_anal_spin<2>, grid, block, 0, st, a); break;
+    default: SHP_FAIL("bad transform group");
+    }
+  }
+  check_launch("analysis");
+}
+
+} // namespace shp

@@ -1,0 +1,304 @@
+// ring_dft.hip -- ring stage by direct summation, and a_lm layout kernels.
+//
+// The ring stage evaluates, for every ring of nph pixels at azimuths
+// phi_j = phi0 + 2 pi j / nph,
+//   synthesis  f_j = Re c_0 + 2 sum_{m=1}^{mmax} Re(c_m e^{i m phi_j}),
+//              c_m = w * phase_m,
+//   analysis   phase_m = w e^{-i m phi0} sum_j f_j e^{-2 pi i m j / nph},
+// which are exactly the sums libsharp forms with its aliasing fold and real
+// FFTs (libsharp/sharp.c:253-353).  This direct form costs O(nph * mmax) per
+// ring; it is exact for every ring length and serves as the reference path
+// of the batched ring FFT in ring_fft.hip.  Twiddles are anchored every 32
+// steps on an exactly range-reduced angle (integer (m*j) mod nph) and advanced
+// by complex multiplication in between.
+//
+// The a_lm kernels move coefficients between the caller's layout
+// (mvstart/stride, SHARP_PACKED, float or double) and the canonical almtmp,
+// applying norm_l and the REAL_HARMONICS m=0 factor
+// (libsharp/sharp.c:481-606).
+
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+#include "sharp_lowlevel.h"
+
+namespace shp {
+
+constexpr double kSqrtHalf = 0.707106781186547572737310929369;
+constexpr double kSqrt2 = 1.414213562373095145474621858739;
+
+// e^{2 pi i k / n} for 0 <= k < n
+__device__ inline double2 unit_root(int64_t k, int n)
+{
+  double s, c;
+  sincospi(2.0 * (double)k / (double)n, &s, &c);
+  return make_double2(c, s);
+}
+
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename T> __device__ inline double ld(const void *p, int64_t i)
+{
+  return (double)static_cast<const T *>(p)[i];
+}
+template <typename T> __device__ inline void st(void *p, int64_t i, double v)
+{
+  static_cast<T *>(p)[i] = (T)v;
+}
+
+// ---------------------------------------------------------------------------
+// phase -> ring, direct summation.  grid: (2*nslots, ncomp, pixel tiles)
+// ---------------------------------------------------------------------------
+constexpr int kTileM = 512;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase2ring_dft(RingArgs a)
+{
+  __shared__ double2 cm[kTileM];
+  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[blockIdx.x];
+  const int n = r.nph;
+  if (n <= 0) return;
+  const int j = blockIdx.z * 256 + threadIdx.x;
+  if ((int)(blockIdx.z * 256) >= n) return;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
+  const bool norot = fabs(r.phi0) < 1e-14;
+  double acc = 0.;
+  const double2 step = unit_root(j < n ? j : 0, n);
+  for (int m0 = 0; m0 <= a.mmax; m0 += kTileM) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTileM; i += 256) {
+      const int m = m0 + i;
+      double2 v = make_double2(0., 0.);
+      if (m <= a.mmax) {
+        v = a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
+        v.x *= wgt; v.y *= wgt;
+        if (!norot && m > 0) {
+          double s, c;
+          sincos(m * r.phi0, &s, &c);
+          v = cmul(v, make_double2(c, s));
+        }
+        if (m > 0) { v.x *= 2.; v.y *= 2.; }
+        else v.y = 0.;
+      }
+      cm[i] = v;
+    }
+    __syncthreads();
+    if (j < n) {
+      const int mend = min(kTileM, a.mmax + 1 - m0);
+      for (int i0 = 0; i0 < mend; i0 += 32) {
+        double2 tw = unit_root(((int64_t)(m0 + i0) * j) % n, n);
+        const int iend = min(i0 + 32, mend);
+        for (int i = i0; i < iend; ++i) {
+          acc += cm[i].x * tw.x - cm[i].y * tw.y;
+          tw = cmul(tw, step);
+        }
+      }
+    }
+  }
+  if (j < n) {
+    const int64_t idx = r.ofs + (int64_t)j * r.stride - a.map_base;
+    void *mp = a.maps.p[comp];
+    const double old = (a.flags & SHARP_ADD) ? (a.dp ? ld<double>(mp, idx) : ld<float>(mp, idx)) : 0.;
+    if (a.dp) st<double>(mp, idx, old + acc);
+    else st<float>(mp, idx, old + acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ring -> phase, direct summation.  grid: (2*nslots, ncomp, m tiles)
+// ---------------------------------------------------------------------------
+constexpr int kTileJ = 1024;
+
+__global__ void __launch_bounds__(256) k_ring2phase_dft(RingArgs a)
+{
+  __shared__ double f[kTileJ];
+  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[blockIdx.x];
+  const int n = r.nph;
+  if (n <= 0) return;
+  const int m = blockIdx.z * 256 + threadIdx.x;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
+  const void *mp = a.maps.p[comp];
+  double2 acc = make_double2(0., 0.);
+  // e^{-2 pi i m / n}
+  double2 step = unit_root((int64_t)(n - (m % n)) % n, n);
+  for (int j0 = 0; j0 < n; j0 += kTileJ) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTileJ; i += 256) {
+      const int jj = j0 + i;
+      double v = 0.;
+      if (jj < n) {
+        const int64_t idx = r.ofs + (int64_t)jj * r.stride - a.map_base;
+        v = a.dp ? ld<double>(mp, idx) : ld<float>(mp, idx);
+      }
+      f[i] = v;
+    }
+    __syncthreads();
+    if (m <= a.mmax) {
+      const int jend = min(kTileJ, n - j0);
+      for (int i0 = 0; i0 < jend; i0 += 32) {
+        const int64_t k = ((int64_t)m * (j0 + i0)) % n;
+        double2 tw = unit_root((n - k) % n, n);
+        const int iend = min(i0 + 32, jend);
+        for (int i = i0; i < iend; ++i) {
+          acc.x += f[i] * tw.x;
+          acc.y += f[i] * tw.y;
+          tw = cmul(tw, step);
+        }
+      }
+    }
+  }
+  if (m > a.mmax) return;
+  acc.x *= wgt; acc.y *= wgt;
+  if (fabs(r.phi0) >= 1e-14) {
+    double s, c;
+    sincos(-m * r.phi0, &s, &c);
+    acc = cmul(acc, make_double2(c, s));
+  }
+  a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// SHARP_NO_FFT: the map holds nph = mmax+1 complex values per ring
+// (libsharp/sharp.c:631-669).  grid: (2*nslots, ncomp)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_phase2ring_direct(RingArgs a)
+{
+  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[blockIdx.x];
+  if (r.nph <= 0) return;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.nph * r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
+  void *mp = a.maps.p[comp];
+  for (int m = threadIdx.x; m <= a.mmax; m += 256) {
+    const double2 v = a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
+    const int64_t idx = 2 * (r.ofs + (int64_t)m * r.stride - a.map_base);
+    double ore = 0., oim = 0.;
+    if (a.flags & SHARP_ADD) {
+      ore = a.dp ? ld<double>(mp, idx) : ld<float>(mp, idx);
+      oim = a.dp ? ld<double>(mp, idx + 1) : ld<float>(mp, idx + 1);
+    }
+    if (a.dp) { st<double>(mp, idx, ore + wgt * v.x); st<double>(mp, idx + 1, oim + wgt * v.y); }
+    else { st<float>(mp, idx, ore + wgt * v.x); st<float>(mp, idx + 1, oim + wgt * v.y); }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ring2phase_direct(RingArgs a)
+{
+  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[blockIdx.x];
+  if (r.nph <= 0) return;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.nph * r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
+  const void *mp = a.maps.p[comp];
+  for (int m = threadIdx.x; m <= a.mmax; m += 256) {
+    const int64_t idx = 2 * (r.ofs + (int64_t)m * r.stride - a.map_base);
+    const double re = a.dp ? ld<double>(mp, idx) : ld<float>(mp, idx);
+    const double im = a.dp ? ld<double>(mp, idx + 1) : ld<float>(mp, idx + 1);
+    a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] =
+        make_double2(re * wgt, im * wgt);
+  }
+}
+
+static void check_launch(const char *what)
+{
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "kernel launch failed (%s): %s", what,
+             hipGetErrorString(e));
+    SHP_FAIL(buf);
+  }
+}
+
+void launch_phase2ring_dft(const RingArgs &a, hipStream_t s)
+{
+  if (a.flags & SHARP_NO_FFT) {
+    hipLaunchKernelGGL(k_phase2ring_direct, dim3(2 * a.nslots, a.ncomp), dim3(256), 0, s, a);
+  } else {
+    const dim3 grid(2 * a.nslots, a.ncomp, (a.nphmax + 255) / 256);
+    if (a.dp) hipLaunchKernelGGL(k_phase2ring_dft<double>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_phase2ring_dft<float>, grid, dim3(256), 0, s, a);
+  }
+  check_launch("phase2ring");
+}
+
+void launch_ring2phase_dft(const RingArgs &a, hipStream_t s)
+{
+  if (a.flags & SHARP_NO_FFT) {
+    hipLaunchKernelGGL(k_ring2phase_direct, dim3(2 * a.nslots, a.ncomp), dim3(256), 0, s, a);
+  } else {
+    const dim3 grid(2 * a.nslots, a.ncomp, (a.mmax + 256) / 256);
+    hipLaunchKernelGGL(k_ring2phase_dft, grid, dim3(256), 0, s, a);
+  }
+  check_launch("ring2phase");
+}
+
+// ---------------------------------------------------------------------------
+// a_lm gather / scatter.  grid: (l tiles, mmax+1, ncomp)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
+{
+  const int m = blockIdx.y, comp = blockIdx.z;
+  const int l = m + blockIdx.x * 256 + threadIdx.x;
+  if (l > a.lmax) return;
+  const bool realonly = a.packed && m == 0;
+  int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
+  if (!a.packed) ofs *= 2;
+  if (!realonly) stride *= 2;
+  const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
+  const void *p = a.alms.p[comp];
+  double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
+  double f = a.norm_l[l];
+  if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
+  a.almtmp[(tri_ofs(m, a.lmax) + l - m) * a.ncomp + comp] = make_double2(re * f, im * f);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
+{
+  const int m = blockIdx.y, comp = blockIdx.z;
+  const int l = m + blockIdx.x * 256 + threadIdx.x;
+  if (l > a.lmax) return;
+  const bool realonly = a.packed && m == 0;
+  int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
+  if (!a.packed) ofs *= 2;
+  if (!realonly) stride *= 2;
+  const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
+  void *p = a.alms.p[comp];
+  const double2 x = a.almtmp[(tri_ofs(m, a.lmax) + l - m) * a.ncomp + comp];
+  double f = a.norm_l[l];
+  if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrtHalf;
+  const bool add = (a.flags & SHARP_ADD) != 0;
+  st<T>(p, idx, (add ? ld<T>(p, idx) : 0.) + x.x * f);
+  if (realonly) return;
+  if (m == 0) {
+    if (!add) st<T>(p, idx + 1, 0.);
+  } else {
+    st<T>(p, idx + 1, (add ? ld<T>(p, idx + 1) : 0.) + x.y * f);
+  }
+}
+
+void launch_alm_gather(const AlmArgs &a, hipStream_t s)
+{
+  const dim3 grid((a.lmax + 256) / 256, a.mmax + 1, a.ncomp);
+  if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_gather<double>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_alm_gather<float>, grid, dim3(256), 0, s, a);
+  check_launch("alm_gather");
+}
+
+void launch_alm_scatter(const AlmArgs &a, hipStream_t s)
+{
+  const dim3 grid((a.lmax + 256) / 256, a.mmax + 1, a.ncomp);
+  if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_scatter<double>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_alm_scatter<float>, grid, dim3(256), 0, s, a);
+  check_launch("alm_scatter");
+}
+
+} // namespace shp

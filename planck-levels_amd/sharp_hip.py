@@ -1,0 +1,291 @@
+"""Python mirror of libsharp's interface over the MI355X engine (ctypes).
+
+The native library ``libsharp_hip.so`` (built from ``csrc/``) exports the
+libsharp C interface unchanged (libsharp/sharp_lowlevel.h,
+sharp_geomhelpers.h, sharp_almhelpers.h) plus the device-resident entry points
+of ``include/sharp_hip.h``.  This module binds both:
+
+* ``Geometry`` / ``AlmInfo`` own ``sharp_geom_info`` / ``sharp_alm_info``
+  objects built by the library's own constructors;
+* ``SharpJob`` mirrors the C++ wrapper ``sharp_cxxjob<T>`` of
+  libsharp/sharp_cxx.h:102-149 (alm2map, alm2map_spin, alm2map_der1, map2alm,
+  map2alm_spin) on numpy arrays in host memory;
+* ``execute`` is ``sharp_execute`` on host arrays and ``execute_device`` its
+  device-resident form on torch CUDA tensors (no host round trip).
+
+If the library is missing this module raises at import: there is no CPU
+fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsharp_hip.so")
+
+# job types and flags (libsharp/sharp_lowlevel.h:162-191, include/sharp_hip.h)
+MAP2ALM, ALM2MAP, Yt, WY, ALM2MAP_DERIV1 = 0, 1, 2, 3, 4
+DP, ADD, REAL_HARMONICS, NO_FFT = 1 << 4, 1 << 5, 1 << 6, 1 << 7
+MIXED_FP32, MIXED_BF16X3 = 1 << 24, 1 << 25
+PACKED = 1
+
+
+class RingInfo(ct.Structure):
+    _fields_ = [("theta", ct.c_double), ("phi0", ct.c_double),
+                ("weight", ct.c_double), ("cth", ct.c_double),
+                ("sth", ct.c_double), ("ofs", ct.c_ssize_t),
+                ("nph", ct.c_int), ("stride", ct.c_int)]
+
+
+class RingPair(ct.Structure):
+    _fields_ = [("r1", RingInfo), ("r2", RingInfo)]
+
+
+class GeomInfoC(ct.Structure):
+    _fields_ = [("pair", ct.POINTER(RingPair)), ("npairs", ct.c_int),
+                ("nphmax", ct.c_int)]
+
+
+class AlmInfoC(ct.Structure):
+    _fields_ = [("lmax", ct.c_int), ("nm", ct.c_int),
+                ("mval", ct.POINTER(ct.c_int)), ("flags", ct.c_int),
+                ("mvstart", ct.POINTER(ct.c_ssize_t)),
+                ("stride", ct.c_ssize_t)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libsharp_hip.so is not built (run __graft_entry__.build() or "
+            "make -C planck-levels_amd/csrc); there is no CPU fallback")
+    lib = ct.CDLL(LIB_PATH, mode=ct.RTLD_LOCAL)
+    G = ct.POINTER(GeomInfoC)
+    A = ct.POINTER(AlmInfoC)
+    lib.sharp_make_geom_info.argtypes = [
+        ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+        ct.c_void_p, ct.c_void_p, ct.POINTER(G)]
+    lib.sharp_make_weighted_healpix_geom_info.argtypes = [
+        ct.c_int, ct.c_int, ct.c_void_p, ct.POINTER(G)]
+    for g in ("gauss", "fejer1", "cc", "fejer2", "mw"):
+        getattr(lib, "sharp_make_%s_geom_info" % g).argtypes = [
+            ct.c_int, ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.POINTER(G)]
+    lib.sharp_destroy_geom_info.argtypes = [G]
+    lib.sharp_make_triangular_alm_info.argtypes = [
+        ct.c_int, ct.c_int, ct.c_int, ct.POINTER(A)]
+    lib.sharp_make_rectangular_alm_info.argtypes = [
+        ct.c_int, ct.c_int, ct.c_int, ct.POINTER(A)]
+    lib.sharp_make_general_alm_info.argtypes = [
+        ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int,
+        ct.POINTER(A)]
+    lib.sharp_destroy_alm_info.argtypes = [A]
+    lib.sharp_alm_index.argtypes = [A, ct.c_int, ct.c_int]
+    lib.sharp_alm_index.restype = ct.c_ssize_t
+    lib.sharp_execute.argtypes = [
+        ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int,
+        ct.POINTER(ct.c_double), ct.POINTER(ct.c_ulonglong)]
+    lib.sharp_hip_execute_device.argtypes = [
+        ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int,
+        ct.c_void_p, ct.POINTER(ct.c_ulonglong)]
+    for f in ("sharp_hip_alm2map_adjoint_device",
+              "sharp_hip_map2alm_adjoint_device"):
+        getattr(lib, f).argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, G, A,
+                                    ct.c_int, ct.c_int, ct.c_void_p]
+    lib.sharp_hip_prepare.argtypes = [G, A, ct.c_int]
+    lib.sharp_hip_synchronize.argtypes = [ct.c_void_p]
+    lib.sharp_hip_device_count.restype = ct.c_int
+    lib.sharp_get_mlim.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double]
+    lib.sharp_get_mlim.restype = ct.c_int
+    return lib
+
+
+LIB = _load()
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class Geometry:
+    """Owns a sharp_geom_info built by the library (sharp_geomhelpers.h)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    @classmethod
+    def healpix(cls, nside, weight=None, stride=1):
+        g = ct.POINTER(GeomInfoC)()
+        w = None if weight is None else _np_ptr(np.ascontiguousarray(weight, np.float64))
+        LIB.sharp_make_weighted_healpix_geom_info(nside, stride, w, ct.byref(g))
+        return cls(g)
+
+    @classmethod
+    def iso(cls, kind, nrings, nphi, phi0=0.0, stride_lon=1, stride_lat=None):
+        """kind: 'gauss', 'fejer1' (= ECP), 'cc', 'fejer2', 'mw'."""
+        g = ct.POINTER(GeomInfoC)()
+        getattr(LIB, "sharp_make_%s_geom_info" % kind)(
+            nrings, nphi, phi0, stride_lon,
+            nphi if stride_lat is None else stride_lat, ct.byref(g))
+        return cls(g)
+
+    @classmethod
+    def general(cls, nph, ofs, stride, phi0, theta, wgt=None):
+        g = ct.POINTER(GeomInfoC)()
+        arrs = [np.ascontiguousarray(nph, np.int32), np.ascontiguousarray(ofs, np.int64),
+                np.ascontiguousarray(stride, np.int32), np.ascontiguousarray(phi0, np.float64),
+                np.ascontiguousarray(theta, np.float64)]
+        w = None if wgt is None else np.ascontiguousarray(wgt, np.float64)
+        LIB.sharp_make_geom_info(len(arrs[0]), *[_np_ptr(a) for a in arrs],
+                                 None if w is None else _np_ptr(w), ct.byref(g))
+        g._keep = (arrs, w)
+        return cls(g)
+
+    @property
+    def npix(self):
+        gi = self.ptr.contents
+        n = 0
+        for i in range(gi.npairs):
+            p = gi.pair[i]
+            n += p.r1.nph + (p.r2.nph if p.r2.nph > 0 else 0)
+        return n
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            LIB.sharp_destroy_geom_info(self.ptr)
+            self.ptr = None
+
+
+class AlmInfo:
+    """Owns a sharp_alm_info (sharp_almhelpers.h / sharp_lowlevel.h)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    @classmethod
+    def triangular(cls, lmax, mmax=None, stride=1):
+        a = ct.POINTER(AlmInfoC)()
+        LIB.sharp_make_triangular_alm_info(lmax, lmax if mmax is None else mmax,
+                                           stride, ct.byref(a))
+        return cls(a)
+
+    @classmethod
+    def rectangular(cls, lmax, mmax=None, stride=1):
+        a = ct.POINTER(AlmInfoC)()
+        LIB.sharp_make_rectangular_alm_info(lmax, lmax if mmax is None else mmax,
+                                            stride, ct.byref(a))
+        return cls(a)
+
+    @classmethod
+    def general(cls, lmax, mval, mvstart, stride=1, flags=0):
+        a = ct.POINTER(AlmInfoC)()
+        mval = np.ascontiguousarray(mval, np.int32)
+        mvstart = np.ascontiguousarray(mvstart, np.int64)
+        LIB.sharp_make_general_alm_info(lmax, len(mval), stride, _np_ptr(mval),
+                                        _np_ptr(mvstart), flags, ct.byref(a))
+        return cls(a)
+
+    @property
+    def lmax(self):
+        return self.ptr.contents.lmax
+
+    @property
+    def mmax(self):
+        return self.ptr.contents.nm - 1
+
+    def index(self, l, mi):
+        return LIB.sharp_alm_index(self.ptr, l, mi)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            LIB.sharp_destroy_alm_info(self.ptr)
+            self.ptr = None
+
+
+def nalm_triangular(lmax, mmax=None):
+    mmax = lmax if mmax is None else mmax
+    return ((mmax + 1) * (mmax + 2)) // 2 + (mmax + 1) * (lmax - mmax)
+
+
+def execute(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags):
+    """sharp_execute on numpy arrays (modified in place).
+
+    Returns (wall time in s, operation count of libsharp's flop model)."""
+    ap = (ct.c_void_p * len(alms))(*[a.ctypes.data for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.ctypes.data for m in maps])
+    t = ct.c_double(0.0)
+    op = ct.c_ulonglong(0)
+    LIB.sharp_execute(jobtype, spin, ap, mp, geom.ptr, ainfo.ptr, ntrans, flags,
+                      ct.byref(t), ct.byref(op))
+    return t.value, op.value
+
+
+def execute_device(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags,
+                   stream=None):
+    """Device-resident sharp_execute on torch CUDA tensors.
+
+    alms: complex128/complex64 tensors (or real views thereof) in the layout
+    described by ainfo; maps: float64/float32 tensors.  Work is enqueued on
+    `stream` (default: torch's current stream); returns the operation count."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+    op = ct.c_ulonglong(0)
+    LIB.sharp_hip_execute_device(jobtype, spin, ap, mp, geom.ptr, ainfo.ptr,
+                                 ntrans, flags, ct.c_void_p(stream.cuda_stream),
+                                 ct.byref(op))
+    return op.value
+
+
+class SharpJob:
+    """Mirror of sharp_cxxjob<T> (libsharp/sharp_cxx.h:39-149) on numpy arrays;
+    dtype float64 (SHARP_DP) or float32."""
+
+    def __init__(self, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.geom = None
+        self.ainfo = None
+
+    def _flags(self, add):
+        return (DP if self.dtype == np.float64 else 0) | (ADD if add else 0)
+
+    def set_Healpix_geometry(self, nside):
+        self.geom = Geometry.healpix(nside)
+
+    def set_weighted_Healpix_geometry(self, nside, weight):
+        self.geom = Geometry.healpix(nside, weight)
+
+    def set_Gauss_geometry(self, nrings, nphi):
+        self.geom = Geometry.iso("gauss", nrings, nphi)
+
+    def set_ECP_geometry(self, nrings, nphi):
+        self.geom = Geometry.iso("fejer1", nrings, nphi)
+
+    def set_general_geometry(self, nph, ofs, stride, phi0, theta, wgt=None):
+        self.geom = Geometry.general(nph, ofs, stride, phi0, theta, wgt)
+
+    def set_triangular_alm_info(self, lmax, mmax):
+        self.ainfo = AlmInfo.triangular(lmax, mmax)
+
+    def alm2map(self, alm, map_, add=False):
+        execute(ALM2MAP, 0, [alm], [map_], self.geom, self.ainfo, 1, self._flags(add))
+
+    def alm2map_spin(self, alm1, alm2, map1, map2, spin, add=False):
+        execute(ALM2MAP, spin, [alm1, alm2], [map1, map2], self.geom, self.ainfo,
+                1, self._flags(add))
+
+    def alm2map_der1(self, alm, map1, map2, add=False):
+        execute(ALM2MAP_DERIV1, 1, [alm], [map1, map2], self.geom, self.ainfo, 1,
+                self._flags(add))
+
+    def map2alm(self, map_, alm, add=False):
+        execute(MAP2ALM, 0, [alm], [map_], self.geom, self.ainfo, 1, self._flags(add))
+
+    def map2alm_spin(self, map1, map2, alm1, alm2, spin, add=False):
+        execute(MAP2ALM, spin, [alm1, alm2], [map1, map2], self.geom, self.ainfo,
+                1, self._flags(add))
+
+
+def device_count():
+    return LIB.sharp_hip_device_count()
