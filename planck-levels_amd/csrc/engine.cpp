@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -97,10 +98,19 @@ struct GeomPlan {
     int *d = nullptr;
   };
   std::vector<Mlim> mlims;
+  // ring transforms: FFT plans grouped by LDS class, the rings too long for
+  // any class (direct summation), and the list of all rings (SHARP_NO_FFT)
+  RingFftPlan *fft[2 * kNumFftClasses] = {};   // [2*class + bluestein]
+  int nfft[2 * kNumFftClasses] = {};
+  int *dft_list = nullptr, ndft = 0;
+  int *all_list = nullptr, nall = 0;
   ~GeomPlan()
   {
     (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
     for (auto &m : mlims) (void)hipFree(m.d);
+    for (auto *p : fft) (void)hipFree(p);
+    (void)hipFree(dft_list);
+    (void)hipFree(all_list);
   }
 };
 
@@ -130,6 +140,7 @@ struct AlmPlan {
 
 struct Ctx {
   int device = 0;
+  int max_lds = 65536;       // bytes of LDS one workgroup may allocate
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<GeomPlan>> geoms;
   std::vector<std::unique_ptr<YlmPlan>> ylms;
@@ -153,9 +164,57 @@ static Ctx &context()
   auto c = std::make_unique<Ctx>();
   c->device = dev;
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess && lds > 0)
+    c->max_lds = lds;
   Ctx &ref = *c;
   g_ctx[dev] = std::move(c);
   return ref;
+}
+
+// --- ring FFT plans -----------------------------------------------------------
+// Radices for a length: powers of 8, then one 4 or 2, then 3, 5, 7.  Returns
+// the cofactor left over (1 when the length is 7-smooth).
+static int radices(int N, std::vector<int> &rad)
+{
+  rad.clear();
+  int n = N;
+  while (n % 8 == 0) { rad.push_back(8); n /= 8; }
+  while (n % 4 == 0) { rad.push_back(4); n /= 4; }
+  while (n % 2 == 0) { rad.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7})
+    while (n % p == 0) { rad.push_back(p); n /= p; }
+  return n;
+}
+
+// Plan for a ring of n pixels; returns its LDS class (buffer of 512<<class
+// complex doubles) or -1 when no class is large enough.
+static int fft_plan(int rid, int n, RingFftPlan &pl)
+{
+  std::memset(&pl, 0, sizeof(pl));
+  pl.ring = rid;
+  pl.n = n;
+  pl.N = (n & 1) ? n : n / 2;
+  std::vector<int> rad;
+  size_t need;
+  if (radices(pl.N, rad) == 1) {
+    pl.bluestein = 0;
+    need = (size_t)pl.N;
+  } else {
+    pl.bluestein = 1;
+    size_t M = 1;
+    while (M < 2 * (size_t)pl.N - 1) M <<= 1;
+    pl.M = (int)std::min<size_t>(M, 1u << 30);
+    radices(pl.M, rad);
+    need = M;
+  }
+  if (rad.size() > sizeof(pl.rad)) return -1;
+  pl.nrad = (int)rad.size();
+  for (size_t i = 0; i < rad.size(); ++i) pl.rad[i] = (int8_t)rad[i];
+  for (int cl = 0; cl < kNumFftClasses; ++cl)
+    if (need <= (size_t)(512 << cl)) return cl;
+  return -1;
 }
 
 // --- geometry ---------------------------------------------------------------
@@ -216,6 +275,28 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     nphmax = std::max(nphmax, std::max(pr.r1.nph, pr.r2.nph));
   }
   gp->nphmax = nphmax;
+  // ring FFT plans by LDS class; rings beyond every class use direct sums
+  std::vector<RingFftPlan> cls[2 * kNumFftClasses];
+  std::vector<int> dft, all;
+  for (int rid = 0; rid < 2 * P; ++rid) {
+    const int n = ring[rid].nph;
+    if (n <= 0) continue;
+    all.push_back(rid);
+    RingFftPlan pl;
+    int cl = fft_plan(rid, n, pl);
+    if (cl >= 0 && (size_t)(512 << cl) * sizeof(double2) + 64 > (size_t)c.max_lds)
+      cl = -1;
+    if (cl < 0) dft.push_back(rid);
+    else cls[2 * cl + pl.bluestein].push_back(pl);
+  }
+  for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
+    gp->nfft[c2] = (int)cls[c2].size();
+    if (!cls[c2].empty()) gp->fft[c2] = upload(cls[c2]);
+  }
+  gp->ndft = (int)dft.size();
+  gp->dft_list = upload(dft);
+  gp->nall = (int)all.size();
+  gp->all_list = upload(all);
   gp->cth = upload(gp->hcth);
   gp->sth = upload(gp->hsth);
   gp->ispair = upload(ispair);
@@ -527,12 +608,41 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
     }
   };
 
+  FftClasses fc{};
+  for (int cl = 0; cl < 2 * kNumFftClasses; ++cl) {
+    fc.plans[cl] = gp.fft[cl];
+    fc.count[cl] = gp.nfft[cl];
+  }
+  const bool nofft = (j.flags & SHARP_NO_FFT) != 0;
+  // SHARP_HIP_RING_DFT=1 routes every ring through the direct-summation
+  // kernels (a cross-check of the FFT path; slow for long rings)
+  static const bool force_dft = [] {
+    const char *e = std::getenv("SHARP_HIP_RING_DFT");
+    return e && e[0] == '1';
+  }();
+  auto rings_to_phase = [&]() {
+    if (nofft || force_dft) {
+      launch_ring2phase_dft(ra, gp.all_list, gp.nall, st);
+    } else {
+      launch_ring_fft(false, ra, fc, st);
+      launch_ring2phase_dft(ra, gp.dft_list, gp.ndft, st);
+    }
+  };
+  auto phase_to_rings = [&]() {
+    if (nofft || force_dft) {
+      launch_phase2ring_dft(ra, gp.all_list, gp.nall, st);
+    } else {
+      launch_ring_fft(true, ra, fc, st);
+      launch_phase2ring_dft(ra, gp.dft_list, gp.ndft, st);
+    }
+  };
+
   if (j.type == SHARP_ALM2MAP) {
     launch_alm_gather(aa, st);
     for_groups([&](int g) { launch_synth(la, g, st); });
-    launch_phase2ring_dft(ra, st);
+    phase_to_rings();
   } else {
-    launch_ring2phase_dft(ra, st);
+    rings_to_phase();
     for_groups([&](int g) { launch_anal(la, g, st); });
     launch_alm_scatter(aa, st);
   }

@@ -48,8 +48,30 @@ struct RingArgs {
   int dp;               // 1: double maps, 0: float maps
   PtrTab maps;          // [ncomp] device pointers (float or double)
 };
-void launch_phase2ring_dft(const RingArgs &a, hipStream_t st);
-void launch_ring2phase_dft(const RingArgs &a, hipStream_t st);
+// Direct-summation ring transforms over the rings in `list` (nlist entries of
+// slot*2+hemi); exact for any ring length, O(nph*mmax) per ring.  Used for
+// SHARP_NO_FFT and for rings too long for the LDS FFT.
+void launch_phase2ring_dft(const RingArgs &a, const int *list, int nlist, hipStream_t st);
+void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStream_t st);
+
+// Per-ring FFT plan.  rad[] lists the Stockham radices of N (smooth lengths)
+// or of the Bluestein length M.
+struct RingFftPlan {
+  int ring;        // slot*2 + hemi
+  int n;           // pixels
+  int N;           // complex FFT length: n/2 (even n) or n (odd n)
+  int bluestein;   // 1: N has a prime factor > 7
+  int M;           // Bluestein convolution length (power of two >= 2N-1)
+  int nrad;
+  int8_t rad[16];
+};
+constexpr int kNumFftClasses = 5;   // LDS buffers of 512 << c complex doubles
+// plans[2*class + bluestein]
+struct FftClasses {
+  const RingFftPlan *plans[2 * kNumFftClasses];
+  int count[2 * kNumFftClasses];
+};
+void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st);
 
 // a_lm in the caller's layout <-> canonical almtmp.
 struct AlmArgs {

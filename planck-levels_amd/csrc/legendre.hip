@@ -493,9 +493,11 @@ __global__ void __launch_bounds__(256) k_anal_spin0(LegArgs a)
     const double mf = a.mfac[m];
     lam2 *= (m & 1) ? -mf : mf;
     normalize(lam2, scale, kFTol);
-    // 0: advancing below the IEEE range, 1: scaled, 2: unscaled, 3: idle
-    int mode = wave_any(active) ? 0 : 3;
-    double cf = 0.;
+    // 0: advancing below the IEEE range, 1: scaled, 2: unscaled, 3: idle.
+    // As in libsharp the advance phase only runs while every lane is below
+    // the IEEE range (libsharp/sharp_core_inc.c:192-205).
+    int mode = !wave_any(active) ? 3 : (wave_all(scale < 1) ? 0 : 1);
+    double cf = corfac(scale);
     int l = m;
 
     for (int c = 0; c < nchunks; ++c) {
@@ -641,8 +643,8 @@ __global__ void __launch_bounds__(256) k_anal_spin(LegArgs a)
     int scalep, scalem;
     spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep,
               scalem);
-    int mode = wave_any(active) ? 0 : 3;
-    double cfp = 0., cfm = 0.;
+    int mode = !wave_any(active) ? 3 : (wave_all(scalep < 1 && scalem < 1) ? 0 : 1);
+    double cfp = corfac(scalep), cfm = corfac(scalem);
     int l = st.mhi;
 
     // a(E,B) += saddstep2(px, py, rp, rm) (libsharp/sharp_core_inc2.c:339-364)

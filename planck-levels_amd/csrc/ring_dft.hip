@@ -54,11 +54,12 @@ template <typename T> __device__ inline void st(void *p, int64_t i, double v)
 constexpr int kTileM = 512;
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_phase2ring_dft(RingArgs a)
+__global__ void __launch_bounds__(256) k_phase2ring_dft(RingArgs a, const int *list)
 {
   __shared__ double2 cm[kTileM];
-  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
-  const RingDev r = a.ring[blockIdx.x];
+  const int rid = list[blockIdx.x];
+  const int slot = rid >> 1, hemi = rid & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[rid];
   const int n = r.nph;
   if (n <= 0) return;
   const int j = blockIdx.z * 256 + threadIdx.x;
@@ -113,11 +114,12 @@ __global__ void __launch_bounds__(256) k_phase2ring_dft(RingArgs a)
 // ---------------------------------------------------------------------------
 constexpr int kTileJ = 1024;
 
-__global__ void __launch_bounds__(256) k_ring2phase_dft(RingArgs a)
+__global__ void __launch_bounds__(256) k_ring2phase_dft(RingArgs a, const int *list)
 {
   __shared__ double f[kTileJ];
-  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
-  const RingDev r = a.ring[blockIdx.x];
+  const int rid = list[blockIdx.x];
+  const int slot = rid >> 1, hemi = rid & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[rid];
   const int n = r.nph;
   if (n <= 0) return;
   const int m = blockIdx.z * 256 + threadIdx.x;
@@ -167,10 +169,11 @@ __global__ void __launch_bounds__(256) k_ring2phase_dft(RingArgs a)
 // SHARP_NO_FFT: the map holds nph = mmax+1 complex values per ring
 // (libsharp/sharp.c:631-669).  grid: (2*nslots, ncomp)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_phase2ring_direct(RingArgs a)
+__global__ void __launch_bounds__(256) k_phase2ring_direct(RingArgs a, const int *list)
 {
-  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
-  const RingDev r = a.ring[blockIdx.x];
+  const int rid = list[blockIdx.x];
+  const int slot = rid >> 1, hemi = rid & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[rid];
   if (r.nph <= 0) return;
   double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.nph * r.weight : 1.;
   if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
@@ -188,10 +191,11 @@ __global__ void __launch_bounds__(256) k_phase2ring_direct(RingArgs a)
   }
 }
 
-__global__ void __launch_bounds__(256) k_ring2phase_direct(RingArgs a)
+__global__ void __launch_bounds__(256) k_ring2phase_direct(RingArgs a, const int *list)
 {
-  const int slot = blockIdx.x >> 1, hemi = blockIdx.x & 1, comp = blockIdx.y;
-  const RingDev r = a.ring[blockIdx.x];
+  const int rid = list[blockIdx.x];
+  const int slot = rid >> 1, hemi = rid & 1, comp = blockIdx.y;
+  const RingDev r = a.ring[rid];
   if (r.nph <= 0) return;
   double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.nph * r.weight : 1.;
   if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
@@ -216,25 +220,27 @@ static void check_launch(const char *what)
   }
 }
 
-void launch_phase2ring_dft(const RingArgs &a, hipStream_t s)
+void launch_phase2ring_dft(const RingArgs &a, const int *list, int nlist, hipStream_t s)
 {
+  if (nlist <= 0) return;
   if (a.flags & SHARP_NO_FFT) {
-    hipLaunchKernelGGL(k_phase2ring_direct, dim3(2 * a.nslots, a.ncomp), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_phase2ring_direct, dim3(nlist, a.ncomp), dim3(256), 0, s, a, list);
   } else {
-    const dim3 grid(2 * a.nslots, a.ncomp, (a.nphmax + 255) / 256);
-    if (a.dp) hipLaunchKernelGGL(k_phase2ring_dft<double>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_phase2ring_dft<float>, grid, dim3(256), 0, s, a);
+    const dim3 grid(nlist, a.ncomp, (a.nphmax + 255) / 256);
+    if (a.dp) hipLaunchKernelGGL(k_phase2ring_dft<double>, grid, dim3(256), 0, s, a, list);
+    else hipLaunchKernelGGL(k_phase2ring_dft<float>, grid, dim3(256), 0, s, a, list);
   }
   check_launch("phase2ring");
 }
 
-void launch_ring2phase_dft(const RingArgs &a, hipStream_t s)
+void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStream_t s)
 {
+  if (nlist <= 0) return;
   if (a.flags & SHARP_NO_FFT) {
-    hipLaunchKernelGGL(k_ring2phase_direct, dim3(2 * a.nslots, a.ncomp), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ring2phase_direct, dim3(nlist, a.ncomp), dim3(256), 0, s, a, list);
   } else {
-    const dim3 grid(2 * a.nslots, a.ncomp, (a.mmax + 256) / 256);
-    hipLaunchKernelGGL(k_ring2phase_dft, grid, dim3(256), 0, s, a);
+    const dim3 grid(nlist, a.ncomp, (a.mmax + 256) / 256);
+    hipLaunchKernelGGL(k_ring2phase_dft, grid, dim3(256), 0, s, a, list);
   }
   check_launch("ring2phase");
 }

@@ -1,0 +1,553 @@
+// ring_fft.hip -- batched ring FFTs for the ragged rings of a map.
+//
+// One workgroup transforms one ring (for one or several components).  The
+// ring's data never leaves LDS between loading and storing:
+//
+//   synthesis (phase -> ring), libsharp/sharp.c:253-303:
+//     1. fold   c_m = w e^{i m phi0} phase_m  (m = 0..mmax) onto the
+//               Hermitian bins C_k (k = m mod n, conjugated for -m): the
+//               aliasing fold of rings with n < 2 mmax + 1;
+//     2. pack   even n: the half-length complex sequence whose inverse FFT
+//               interleaves even and odd pixels; odd n: the full Hermitian
+//               sequence;
+//     3. FFT    inverse complex FFT of length N (= n/2 or n) in LDS;
+//     4. store  pixels (added to the map with SHARP_ADD).
+//   analysis (ring -> phase), libsharp/sharp.c:305-353: the reverse, ending
+//     with phase_m = w e^{-i m phi0} X_{m mod n} (conjugated past n/2).
+//
+// Complex FFTs of length N are Stockham autosort passes of radix 8, 4, 2, 3,
+// 5 and 7, in place in one LDS buffer (every pass reads all its inputs into
+// registers, barriers, then writes).  A length with another prime factor is
+// done by Bluestein's algorithm: a circular convolution of length
+// M = 2^k >= 2N-1 with the chirp e^{-i pi j^2 / N}; the chirp's FFT is formed
+// once per ring and kept in registers for all components.
+//
+// Rings are grouped by the LDS buffer they need (512 ... 8192 complex
+// doubles); each group is one launch of a kernel instantiated for that size
+// with S/8 threads, so every thread carries at most 8 complex values per pass.
+
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+#include "sharp_lowlevel.h"
+#include "devmath.hpp"
+
+namespace shp {
+
+namespace {
+
+constexpr double kSqrtHalf = 0.707106781186547572737310929369;
+constexpr double kSqrt2 = 1.414213562373095145474621858739;
+
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 conj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// multiply by sign*i
+__device__ inline double2 mul_si(double2 a, int sign)
+{
+  return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+// e^{i pi x} for |x| <= 2
+__device__ inline double2 expipi(double x)
+{
+  return expi_small(3.141592653589793 * x);
+}
+
+template <typename T> __device__ inline double ld(const void *p, int64_t i)
+{
+  return (double)static_cast<const T *>(p)[i];
+}
+template <typename T> __device__ inline void st(void *p, int64_t i, double v)
+{
+  static_cast<T *>(p)[i] = (T)v;
+}
+
+// --- small DFTs: X_q = sum_r v_r e^{sign 2 pi i r q / R} ---------------------
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+  static __device__ inline void run(double2 *v, int)
+  {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <> struct Dft<4> {
+  static __device__ inline void run(double2 *v, int sign)
+  {
+    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const double2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <> struct Dft<8> {
+  static __device__ inline void run(double2 *v, int sign)
+  {
+    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4>::run(e, sign);
+    Dft<4>::run(o, sign);
+    // o_k *= e^{sign 2 pi i k / 8}
+    const double h = kSqrtHalf;
+    o[1] = cmul(o[1], make_double2(h, sign * h));
+    o[2] = mul_si(o[2], sign);
+    o[3] = cmul(o[3], make_double2(-h, sign * h));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+template <> struct Dft<3> {
+  static __device__ inline void run(double2 *v, int sign)
+  {
+    const double s3 = 0.866025403784438646763723170752936183;  // sqrt(3)/2
+    const double2 a = v[0], b = v[1], c = v[2];
+    const double2 bc = cadd(b, c);
+    const double2 t = make_double2(a.x - 0.5 * bc.x, a.y - 0.5 * bc.y);
+    const double2 u = mul_si(cscale(csub(b, c), s3), sign);
+    v[0] = cadd(a, bc);
+    v[1] = cadd(t, u);
+    v[2] = csub(t, u);
+  }
+};
+
+// odd prime radix R = 5, 7 by the symmetric pairing of r and R-r:
+//   X_q, X_{R-q} = v_0 + sum_r (v_r + v_{R-r}) cos(2 pi r q / R)
+//                  +- sign i sum_r (v_r - v_{R-r}) sin(2 pi r q / R)
+struct Tw5 {
+  static constexpr double c[5] = {1.0, 0.30901699437494745, -0.8090169943749475,
+                                  -0.8090169943749475, 0.30901699437494745};
+  static constexpr double s[5] = {0.0, 0.9510565162951535, 0.5877852522924731,
+                                  -0.5877852522924731, -0.9510565162951535};
+};
+struct Tw7 {
+  static constexpr double c[7] = {1.0, 0.6234898018587335, -0.2225209339563144,
+                                  -0.9009688679024191, -0.9009688679024191,
+                                  -0.2225209339563144, 0.6234898018587335};
+  static constexpr double s[7] = {0.0, 0.7818314824680298, 0.9749279121818236,
+                                  0.4338837391175581, -0.4338837391175581,
+                                  -0.9749279121818236, -0.7818314824680298};
+};
+template <int R, typename TW> struct DftOdd {
+  static __device__ inline void run(double2 *v, int sign)
+  {
+    constexpr int H = (R - 1) / 2;
+    double2 a[H + 1], b[H + 1];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      a[r] = cadd(v[r], v[R - r]);
+      b[r] = csub(v[r], v[R - r]);
+    }
+    double2 x0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) x0 = cadd(x0, a[r]);
+    double2 out[R];
+    out[0] = x0;
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+      double2 A = v[0], B = make_double2(0., 0.);
+#pragma unroll
+      for (int r = 1; r <= H; ++r) {
+        const int k = (r * q) % R;
+        A.x = fma(a[r].x, TW::c[k], A.x);
+        A.y = fma(a[r].y, TW::c[k], A.y);
+        B.x = fma(b[r].x, TW::s[k], B.x);
+        B.y = fma(b[r].y, TW::s[k], B.y);
+      }
+      const double2 iB = mul_si(B, sign);
+      out[q] = cadd(A, iB);
+      out[R - q] = csub(A, iB);
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+  }
+};
+template <> struct Dft<5> : DftOdd<5, Tw5> {};
+template <> struct Dft<7> : DftOdd<7, Tw7> {};
+
+// --- one Stockham pass of radix R over buf[0..N) -----------------------------
+template <int NT, int E, int R>
+__device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
+{
+  constexpr int BPT = (E + R - 1) / R;
+  const int tid = threadIdx.x;
+  const int nb = N / R;
+  double2 v[BPT][R];
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int j = tid + i * NT;
+    if (j < nb) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * nb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int j = tid + i * NT;
+    if (j < nb) {
+      const int k = j % Ns;
+      if (Ns > 1) {
+        const double2 w = expipi(sign * 2.0 * k / (double)(Ns * R));
+        double2 wr = w;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[i][r] = cmul(v[i][r], wr);
+          if (r + 1 < R) wr = cmul(wr, w);
+        }
+      }
+      Dft<R>::run(v[i], sign);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// complex FFT of length N (product of the plan radices) in place in LDS.
+// The plan lists the radices grouped as 8..8, then 4 or 2, then 3s, 5s, 7s;
+// the passes are laid out in that order as straight-line code (a switch
+// inside one loop makes the register allocator keep every pass's values
+// live at once).
+template <int NT, int E>
+__device__ void lds_fft(double2 *buf, int N, const int8_t *rad, int nrad, int sign)
+{
+  int Ns = 1, s = 0;
+  for (; s < nrad && rad[s] == 8; ++s, Ns *= 8) stockham_pass<NT, E, 8>(buf, N, Ns, sign);
+  if (s < nrad && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }
+  if (s < nrad && rad[s] == 2) { stockham_pass<NT, E, 2>(buf, N, Ns, sign); Ns *= 2; ++s; }
+  for (; s < nrad && rad[s] == 3; ++s, Ns *= 3) stockham_pass<NT, E, 3>(buf, N, Ns, sign);
+  for (; s < nrad && rad[s] == 5; ++s, Ns *= 5) stockham_pass<NT, E, 5>(buf, N, Ns, sign);
+  for (; s < nrad && rad[s] == 7; ++s, Ns *= 7) stockham_pass<NT, E, 7>(buf, N, Ns, sign);
+}
+
+// Bluestein chirp h_j = e^{sign i pi j^2 / N} with the exact angle
+__device__ inline double2 chirp(int64_t j, int N, int sign)
+{
+  const int64_t q = (j * j) % (2 * (int64_t)N);
+  return expipi(sign * (double)q / (double)N);
+}
+
+// FFT of the convolution kernel b_d = e^{-sign i pi d^2 / N}, |d| < N,
+// wrapped to length M; returns this thread's 8 values (index tid + q*NT).
+template <int NT, int E>
+__device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign,
+                                 double2 (&B)[E])
+{
+  const int N = P.N, M = P.M;
+  for (int j = threadIdx.x; j < M; j += NT) {
+    int d = -1;
+    if (j < N) d = j;
+    else if (j > M - N) d = M - j;
+    buf[j] = (d >= 0) ? chirp(d, N, -sign) : make_double2(0., 0.);
+  }
+  __syncthreads();
+  lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    B[q] = (idx < M) ? buf[idx] : make_double2(0., 0.);
+  }
+  __syncthreads();
+}
+
+// length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein
+template <int NT, int E, bool BLUE>
+__device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign,
+                         const double2 (&B)[BLUE ? E : 1])
+{
+  if constexpr (!BLUE) {
+    lds_fft<NT, E>(buf, P.N, P.rad, P.nrad, sign);
+  } else {
+  const int N = P.N, M = P.M;
+  for (int j = threadIdx.x; j < M; j += NT)
+    buf[j] = (j < N) ? cmul(buf[j], chirp(j, N, sign)) : make_double2(0., 0.);
+  __syncthreads();
+  lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < M) buf[idx] = cmul(buf[idx], B[q]);
+  }
+  __syncthreads();
+  lds_fft<NT, E>(buf, M, P.rad, P.nrad, +1);
+  const double inv = 1.0 / M;
+  for (int j = threadIdx.x; j < N; j += NT)
+    buf[j] = cscale(cmul(buf[j], chirp(j, N, sign)), inv);
+  __syncthreads();
+  }
+}
+
+// reduce a complex value over the G consecutive lanes that share a bin
+__device__ inline double2 group_sum(double2 v, int G)
+{
+  for (int d = 1; d < G; d <<= 1) {
+    v.x += __shfl_xor(v.x, d);
+    v.y += __shfl_xor(v.y, d);
+  }
+  return v;
+}
+
+// threads per ring: S/8 (8 values per thread and pass), at most 512
+template <int S> constexpr int kThreads = (S / 8 < 512) ? S / 8 : 512;
+
+} // namespace
+
+// ===========================================================================
+// synthesis: phase -> ring
+// ===========================================================================
+template <int S, bool BLUE, typename T>
+__global__ void __launch_bounds__(kThreads<S>) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
+                                                          int comps_per_block)
+{
+  constexpr int NT = kThreads<S>, E = S / NT;
+  __shared__ double2 buf[S];
+  __shared__ double2 nyq;
+  const RingFftPlan &P = plans[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int slot = P.ring >> 1, hemi = P.ring & 1;
+  const RingDev r = a.ring[P.ring];
+  const int n = P.n, N = P.N, mmax = a.mmax;
+  const bool even = (n & 1) == 0;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
+  const bool norot = fabs(r.phi0) < 1e-14;
+
+  double2 B[BLUE ? E : 1];
+  if constexpr (BLUE) bluestein_kernel<NT, E>(buf, P, +1, B);
+
+  // bins: even n -> C_0..C_N, odd n -> C_0..C_{(n-1)/2}
+  const int nbins = even ? N + 1 : (n + 1) / 2;
+  int G = 1;
+  while (G < 64 && 2 * G * nbins <= NT) G <<= 1;
+  const int bins_per_round = NT / G;
+
+  const int c0 = blockIdx.y * comps_per_block;
+  const int c1 = min(c0 + comps_per_block, a.ncomp);
+  for (int comp = c0; comp < c1; ++comp) {
+    // 1. fold
+    for (int kb = 0; kb < nbins; kb += bins_per_round) {
+      const int k = kb + tid / G, part = tid % G;
+      double2 s = make_double2(0., 0.);
+      if (k < nbins) {
+        for (int64_t m = k + (int64_t)part * n; m <= mmax; m += (int64_t)G * n) {
+          double2 v = a.phase[phase_index(comp, (int)m, slot, hemi, a.nslots, a.nmb)];
+          v = cscale(v, wgt);
+          if (m == 0) {
+            s.x += v.x;
+          } else {
+            if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
+            s = cadd(s, v);
+          }
+        }
+        for (int64_t m = (int64_t)(1 + part) * n - k; m <= mmax; m += (int64_t)G * n) {
+          if (m < 1) continue;
+          double2 v = a.phase[phase_index(comp, (int)m, slot, hemi, a.nslots, a.nmb)];
+          v = cscale(v, wgt);
+          if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
+          s = cadd(s, conj(v));
+        }
+      }
+      s = group_sum(s, G);
+      if (part == 0 && k < nbins) {
+        if (even) {
+          if (k < N) buf[k] = s;
+          else nyq = s;
+        } else {
+          buf[k] = s;
+          if (k > 0) buf[n - k] = conj(s);
+        }
+      }
+    }
+    __syncthreads();
+    // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
+    if (even) {
+      for (int k = tid; 2 * k <= N; k += NT) {
+        const int kp = N - k;
+        const double2 ck = buf[k], ckp = (k == 0) ? nyq : buf[kp];
+        const double2 e = cadd(ck, conj(ckp));
+        const double2 o = cmul(csub(ck, conj(ckp)), expipi(2.0 * k / n));
+        buf[k] = make_double2(e.x - o.y, e.y + o.x);
+        if (kp < N && kp != k) {
+          const double2 e2 = cadd(ckp, conj(ck));
+          const double2 o2 = cmul(csub(ckp, conj(ck)), expipi(2.0 * kp / n));
+          buf[kp] = make_double2(e2.x - o2.y, e2.y + o2.x);
+        }
+      }
+      __syncthreads();
+    }
+    // 3. inverse FFT of length N
+    ring_dft<NT, E, BLUE>(buf, P, +1, B);
+    // 4. store
+    void *mp = a.maps.p[comp];
+    const bool add = (a.flags & SHARP_ADD) != 0;
+    for (int j = tid; j < N; j += NT) {
+      const double2 z = buf[j];
+      if (even) {
+        const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
+        const int64_t i1 = i0 + r.stride;
+        st<T>(mp, i0, (add ? ld<T>(mp, i0) : 0.) + z.x);
+        st<T>(mp, i1, (add ? ld<T>(mp, i1) : 0.) + z.y);
+      } else {
+        const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
+        st<T>(mp, i0, (add ? ld<T>(mp, i0) : 0.) + z.x);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// analysis: ring -> phase
+// ===========================================================================
+template <int S, bool BLUE, typename T>
+__global__ void __launch_bounds__(kThreads<S>) k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
+                                                          int comps_per_block)
+{
+  constexpr int NT = kThreads<S>, E = S / NT;
+  __shared__ double2 buf[S];
+  __shared__ double2 nyq;
+  const RingFftPlan &P = plans[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int slot = P.ring >> 1, hemi = P.ring & 1;
+  const RingDev r = a.ring[P.ring];
+  const int n = P.n, N = P.N, mmax = a.mmax;
+  const bool even = (n & 1) == 0;
+  double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+  if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
+  const bool norot = fabs(r.phi0) < 1e-14;
+
+  double2 B[BLUE ? E : 1];
+  if constexpr (BLUE) bluestein_kernel<NT, E>(buf, P, -1, B);
+
+  const int c0 = blockIdx.y * comps_per_block;
+  const int c1 = min(c0 + comps_per_block, a.ncomp);
+  for (int comp = c0; comp < c1; ++comp) {
+    const void *mp = a.maps.p[comp];
+    // 1. load: even n -> z_j = f_2j + i f_2j+1, odd n -> z_j = f_j
+    for (int j = tid; j < N; j += NT) {
+      if (even) {
+        const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
+        buf[j] = make_double2(ld<T>(mp, i0), ld<T>(mp, i0 + r.stride));
+      } else {
+        const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
+        buf[j] = make_double2(ld<T>(mp, i0), 0.);
+      }
+    }
+    __syncthreads();
+    // 2. forward FFT
+    ring_dft<NT, E, BLUE>(buf, P, -1, B);
+    // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
+    if (even) {
+      for (int k = tid; 2 * k <= N; k += NT) {
+        const int kp = N - k;
+        if (k == 0) {
+          const double2 z0 = buf[0];
+          buf[0] = make_double2(z0.x + z0.y, 0.);
+          nyq = make_double2(z0.x - z0.y, 0.);
+        } else {
+          const double2 zk = buf[k], zkp = buf[kp];
+          const double2 s1 = cscale(cadd(zk, conj(zkp)), 0.5);
+          const double2 d1 = cmul(cscale(csub(zk, conj(zkp)), 0.5), expipi(-2.0 * k / n));
+          buf[k] = make_double2(s1.x + d1.y, s1.y - d1.x);
+          if (kp != k) {
+            const double2 s2 = cscale(cadd(zkp, conj(zk)), 0.5);
+            const double2 d2 = cmul(cscale(csub(zkp, conj(zk)), 0.5), expipi(-2.0 * kp / n));
+            buf[kp] = make_double2(s2.x + d2.y, s2.y - d2.x);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // 4. phases
+    for (int m = tid; m <= mmax; m += NT) {
+      const int k = m % n;
+      double2 v;
+      if (even) v = (k < N) ? buf[k] : ((k == N) ? nyq : conj(buf[n - k]));
+      else v = buf[k];
+      v = cscale(v, wgt);
+      if (!norot) v = cmul(v, expi_small((double)m * (-r.phi0)));
+      a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// launchers: one launch per LDS size class
+// ===========================================================================
+static void check_launch(const char *what)
+{
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "kernel launch failed (%s): %s", what,
+             hipGetErrorString(e));
+    SHP_FAIL(buf);
+  }
+}
+
+template <int S>
+static void launch_class(bool synth, bool blue, const RingArgs &a,
+                         const RingFftPlan *plans, int nplans, hipStream_t st)
+{
+  // split components over blocks only when there are few rings
+  int cpb = a.ncomp;
+  while (cpb > 1 && (int64_t)nplans * ((a.ncomp + cpb - 1) / cpb) < 2048) cpb = (cpb + 1) / 2;
+  const dim3 grid(nplans, (a.ncomp + cpb - 1) / cpb);
+  const dim3 block(kThreads<S>);
+  if (synth) {
+    if (blue) {
+      if (a.dp) hipLaunchKernelGGL((k_phase2ring_fft<S, true, double>), grid, block, 0, st, a, plans, cpb);
+      else hipLaunchKernelGGL((k_phase2ring_fft<S, true, float>), grid, block, 0, st, a, plans, cpb);
+    } else {
+      if (a.dp) hipLaunchKernelGGL((k_phase2ring_fft<S, false, double>), grid, block, 0, st, a, plans, cpb);
+      else hipLaunchKernelGGL((k_phase2ring_fft<S, false, float>), grid, block, 0, st, a, plans, cpb);
+    }
+  } else {
+    if (blue) {
+      if (a.dp) hipLaunchKernelGGL((k_ring2phase_fft<S, true, double>), grid, block, 0, st, a, plans, cpb);
+      else hipLaunchKernelGGL((k_ring2phase_fft<S, true, float>), grid, block, 0, st, a, plans, cpb);
+    } else {
+      if (a.dp) hipLaunchKernelGGL((k_ring2phase_fft<S, false, double>), grid, block, 0, st, a, plans, cpb);
+      else hipLaunchKernelGGL((k_ring2phase_fft<S, false, float>), grid, block, 0, st, a, plans, cpb);
+    }
+  }
+  check_launch(synth ? "phase2ring_fft" : "ring2phase_fft");
+}
+
+void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st)
+{
+  // largest rings first: they take longest
+  for (int c = kNumFftClasses - 1; c >= 0; --c) {
+    for (int blue = 1; blue >= 0; --blue) {
+      const int idx = 2 * c + blue;
+      if (fc.count[idx] == 0) continue;
+      const RingFftPlan *p = fc.plans[idx];
+      const int np = fc.count[idx];
+      switch (c) {
+      case 0: launch_class<512>(synth, blue, a, p, np, st); break;
+      case 1: launch_class<1024>(synth, blue, a, p, np, st); break;
+      case 2: launch_class<2048>(synth, blue, a, p, np, st); break;
+      case 3: launch_class<4096>(synth, blue, a, p, np, st); break;
+      case 4: launch_class<8192>(synth, blue, a, p, np, st); break;
+      default: break;
+      }
+    }
+  }
+}
+
+} // namespace shp

@@ -55,6 +55,13 @@ class AlmInfoC(ct.Structure):
 
 
 def _load():
+    # One HIP runtime per process: PyTorch's ROCm wheels carry their own
+    # libamdhip64 (with the same soname); when torch is present it is loaded
+    # first, so this library binds to that runtime instead of a second copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libsharp_hip.so is not built (run __graft_entry__.build() or "
