@@ -56,6 +56,18 @@ void sharp_hip_synchronize (void *stream);
 /* Releases every cached table and work buffer of the current device. */
 void sharp_hip_release (void);
 
+/* Per-stage timing (the reference reports only whole-call time and a flop
+   count, libsharp/sharp_lowlevel.h:218-221).  sharp_hip_profile(1) clears the
+   counters and records a pair of HIP events on the launch stream around each
+   stage of every later call; sharp_hip_profile(0) stops.
+   sharp_hip_profile_stats waits for the recorded events and fills, for up to
+   maxn stages: names (32 chars each), total milliseconds, launches, and the
+   algorithmic flops (libsharp's flop model) and compulsory HBM bytes of those
+   launches.  Returns the number of stages. */
+void sharp_hip_profile (int enable);
+int sharp_hip_profile_stats (int maxn, char *names, double *ms,
+  long long *launches, double *flops, double *bytes);
+
 /* Number of visible GPUs (0 on a host without one). */
 int sharp_hip_device_count (void);
 

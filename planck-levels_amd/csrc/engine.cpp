@@ -25,6 +25,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "sharp_lowlevel.h"
@@ -104,6 +105,7 @@ struct GeomPlan {
   int nfft[2 * kNumFftClasses] = {};
   int *dft_list = nullptr, ndft = 0;
   int *all_list = nullptr, nall = 0;
+  int64_t npix_fft = 0, npix_all = 0;   // pixels in FFT rings / in all rings
   ~GeomPlan()
   {
     (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
@@ -138,8 +140,63 @@ struct AlmPlan {
   ~AlmPlan() { (void)hipFree(mvstart_by_m); }
 };
 
+// Opt-in per-stage timing: HIP events recorded on the launch stream around
+// each stage, with the algorithmic work of that launch (libsharp's flop
+// model for the Legendre stage, compulsory HBM bytes for the others).
+struct Profiler {
+  bool on = false;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  struct Stat {
+    std::string name;
+    double ms = 0., flops = 0., bytes = 0.;
+    long long launches = 0;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<Stat> stats;
+  hipEvent_t event()
+  {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  int stage(const char *name)
+  {
+    for (size_t i = 0; i < stats.size(); ++i)
+      if (stats[i].name == name) return (int)i;
+    stats.push_back(Stat{name});
+    return (int)stats.size() - 1;
+  }
+  void drain()
+  {
+    for (auto &p : pending) {
+      HIP_CHECK(hipEventSynchronize(p.b));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+      Stat &st = stats[p.stage];
+      st.ms += ms;
+      st.flops += p.flops;
+      st.bytes += p.bytes;
+      st.launches += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
 struct Ctx {
   int device = 0;
+  Profiler prof;
   int max_lds = 65536;       // bytes of LDS one workgroup may allocate
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<GeomPlan>> geoms;
@@ -286,8 +343,12 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     int cl = fft_plan(rid, n, pl);
     if (cl >= 0 && (size_t)(512 << cl) * sizeof(double2) + 64 > (size_t)c.max_lds)
       cl = -1;
+    gp->npix_all += n;
     if (cl < 0) dft.push_back(rid);
-    else cls[2 * cl + pl.bluestein].push_back(pl);
+    else {
+      cls[2 * cl + pl.bluestein].push_back(pl);
+      gp->npix_fft += n;
+    }
   }
   for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
     gp->nfft[c2] = (int)cls[c2].size();
@@ -528,6 +589,22 @@ static unsigned long long op_count(const Job &j, const GeomPlan &gp,
   return triples * per;
 }
 
+// runs fn() between two events on st when profiling is on
+template <typename F>
+static void timed(Ctx &c, hipStream_t st, const char *name, double flops,
+                  double bytes, F &&fn)
+{
+  if (!c.prof.on) {
+    fn();
+    return;
+  }
+  Profiler::Pending p{c.prof.stage(name), c.prof.event(), c.prof.event(), flops, bytes};
+  HIP_CHECK(hipEventRecord(p.a, st));
+  fn();
+  HIP_CHECK(hipEventRecord(p.b, st));
+  c.prof.pending.push_back(p);
+}
+
 // ---------------------------------------------------------------------------
 // device pipeline
 // ---------------------------------------------------------------------------
@@ -597,22 +674,34 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   la.phase = phase;
 
   const int gmax = (j.spin == 0) ? 4 : 2;
-  auto for_groups = [&](auto &&fn) {
+  // triples (m, ring pair, l) with m <= mlim of the pair, and the flop model
+  // constants of libsharp/sharp_core_inc2.c:171-173,440-442,583-585
+  const double triples = (double)op_count(j, gp, ml) /
+      (double)(j.deriv ? 12 + 8 * j.ntrans : (j.spin == 0 ? 4 + 4 * j.ntrans : 12 + 16 * j.ntrans));
+  const double c0 = j.spin == 0 ? 4. : 12., c1 = j.deriv ? 8. : (j.spin == 0 ? 4. : 16.);
+  const char *syn_name = j.deriv ? "legendre_synth_deriv1"
+                                 : (j.spin == 0 ? "legendre_synth_spin0" : "legendre_synth_spin");
+  const char *ana_name = j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin";
+  auto for_groups = [&](const char *name, auto &&fn) {
     int t = 0;
     while (t < j.ntrans) {
       int g = gmax;
       while (g > j.ntrans - t) g >>= 1;
       la.t0 = t;
-      fn(g);
+      timed(c, st, name, triples * (c0 + c1 * g), 0., [&] { fn(g); });
       t += g;
     }
   };
 
   FftClasses fc{};
+  int64_t nring_fft = 0, npix_fft = 0, npix_all = 0;
   for (int cl = 0; cl < 2 * kNumFftClasses; ++cl) {
     fc.plans[cl] = gp.fft[cl];
     fc.count[cl] = gp.nfft[cl];
+    nring_fft += gp.nfft[cl];
   }
+  npix_fft = gp.npix_fft;
+  npix_all = gp.npix_all;
   const bool nofft = (j.flags & SHARP_NO_FFT) != 0;
   // SHARP_HIP_RING_DFT=1 routes every ring through the direct-summation
   // kernels (a cross-check of the FFT path; slow for long rings)
@@ -620,31 +709,45 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
     const char *e = std::getenv("SHARP_HIP_RING_DFT");
     return e && e[0] == '1';
   }();
+  const double esz = (j.flags & SHARP_DP) ? 8. : 4.;
+  // compulsory traffic of the ring stage: the phases of every ring and its pixels
+  const double ring_bytes_fft =
+      (double)ncomp_map * ((double)nring_fft * (j.mmax + 1) * 16. + (double)npix_fft * esz);
+  const double ring_bytes_all =
+      (double)ncomp_map * ((double)gp.nall * (j.mmax + 1) * 16. + (double)npix_all * esz);
+  const double ring_bytes_dft = std::max(0., ring_bytes_all - ring_bytes_fft);
   auto rings_to_phase = [&]() {
     if (nofft || force_dft) {
-      launch_ring2phase_dft(ra, gp.all_list, gp.nall, st);
+      timed(c, st, "ring_dft_anal", 0., ring_bytes_all,
+            [&] { launch_ring2phase_dft(ra, gp.all_list, gp.nall, st); });
     } else {
-      launch_ring_fft(false, ra, fc, st);
-      launch_ring2phase_dft(ra, gp.dft_list, gp.ndft, st);
+      timed(c, st, "ring_fft_anal", 0., ring_bytes_fft, [&] { launch_ring_fft(false, ra, fc, st); });
+      if (gp.ndft)
+        timed(c, st, "ring_dft_anal", 0., ring_bytes_dft,
+              [&] { launch_ring2phase_dft(ra, gp.dft_list, gp.ndft, st); });
     }
   };
   auto phase_to_rings = [&]() {
     if (nofft || force_dft) {
-      launch_phase2ring_dft(ra, gp.all_list, gp.nall, st);
+      timed(c, st, "ring_dft_synth", 0., ring_bytes_all,
+            [&] { launch_phase2ring_dft(ra, gp.all_list, gp.nall, st); });
     } else {
-      launch_ring_fft(true, ra, fc, st);
-      launch_phase2ring_dft(ra, gp.dft_list, gp.ndft, st);
+      timed(c, st, "ring_fft_synth", 0., ring_bytes_fft, [&] { launch_ring_fft(true, ra, fc, st); });
+      if (gp.ndft)
+        timed(c, st, "ring_dft_synth", 0., ring_bytes_dft,
+              [&] { launch_phase2ring_dft(ra, gp.dft_list, gp.ndft, st); });
     }
   };
+  const double alm_bytes = (double)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm * (16. + 2. * esz);
 
   if (j.type == SHARP_ALM2MAP) {
-    launch_alm_gather(aa, st);
-    for_groups([&](int g) { launch_synth(la, g, st); });
+    timed(c, st, "alm_gather", 0., alm_bytes, [&] { launch_alm_gather(aa, st); });
+    for_groups(syn_name, [&](int g) { launch_synth(la, g, st); });
     phase_to_rings();
   } else {
     rings_to_phase();
-    for_groups([&](int g) { launch_anal(la, g, st); });
-    launch_alm_scatter(aa, st);
+    for_groups(ana_name, [&](int g) { launch_anal(la, g, st); });
+    timed(c, st, "alm_scatter", 0., alm_bytes, [&] { launch_alm_scatter(aa, st); });
   }
 }
 
@@ -802,6 +905,33 @@ void sharp_hip_release(void)
   c.almtmp.release();
   c.stage_alm.release();
   c.stage_map.release();
+}
+
+void sharp_hip_profile(int enable)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Ctx &c = context();
+  c.prof.drain();
+  c.prof.stats.clear();
+  c.prof.on = enable != 0;
+}
+
+int sharp_hip_profile_stats(int maxn, char *names, double *ms, long long *launches,
+                            double *flops, double *bytes)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Ctx &c = context();
+  c.prof.drain();
+  const int n = (int)c.prof.stats.size();
+  for (int i = 0; i < n && i < maxn; ++i) {
+    const auto &s = c.prof.stats[i];
+    std::snprintf(names + 32 * i, 32, "%s", s.name.c_str());
+    ms[i] = s.ms;
+    launches[i] = s.launches;
+    flops[i] = s.flops;
+    bytes[i] = s.bytes;
+  }
+  return n;
 }
 
 int sharp_hip_device_count(void)

@@ -101,6 +101,10 @@ def _load():
     lib.sharp_hip_prepare.argtypes = [G, A, ct.c_int]
     lib.sharp_hip_synchronize.argtypes = [ct.c_void_p]
     lib.sharp_hip_device_count.restype = ct.c_int
+    lib.sharp_hip_profile.argtypes = [ct.c_int]
+    lib.sharp_hip_profile_stats.argtypes = [
+        ct.c_int, ct.c_char_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+    lib.sharp_hip_profile_stats.restype = ct.c_int
     lib.sharp_get_mlim.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double]
     lib.sharp_get_mlim.restype = ct.c_int
     return lib
@@ -296,3 +300,30 @@ class SharpJob:
 
 def device_count():
     return LIB.sharp_hip_device_count()
+
+
+def synchronize(stream=None):
+    LIB.sharp_hip_synchronize(None if stream is None else ct.c_void_p(stream.cuda_stream))
+
+
+def profile(enable):
+    """Start (True, clears counters) or stop per-stage event timing."""
+    LIB.sharp_hip_profile(1 if enable else 0)
+
+
+def profile_stats(maxn=64):
+    """Per-stage totals recorded since profile(True): list of dicts with
+    name, ms, launches, flops (libsharp flop model), bytes (compulsory HBM)."""
+    names = ct.create_string_buffer(32 * maxn)
+    ms = np.zeros(maxn)
+    launches = np.zeros(maxn, np.int64)
+    flops = np.zeros(maxn)
+    byts = np.zeros(maxn)
+    n = LIB.sharp_hip_profile_stats(maxn, names, _np_ptr(ms), _np_ptr(launches),
+                                    _np_ptr(flops), _np_ptr(byts))
+    out = []
+    for i in range(min(n, maxn)):
+        nm = names.raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
+        out.append(dict(name=nm, ms=float(ms[i]), launches=int(launches[i]),
+                        flops=float(flops[i]), bytes=float(byts[i])))
+    return out
