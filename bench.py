@@ -15,7 +15,7 @@ kernels run on); the slowest stage gives the roofline object.  The CPU
 baseline runs the reference libsharp (oracle/_ref, OpenMP) on one realisation
 of the same workload in the same run, when that build is present.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fp64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|fp64]
 """
 import argparse
 import ctypes as ct
@@ -125,6 +125,74 @@ def cpu_baseline(nside, lmax, T0, E0, B0):
     return info, (map_t[0], map_qu[0], map_qu[1]), (out_t[0], out_eb[0], out_eb[1])
 
 
+def run_form(S, torch, step, warmup, steps):
+    """W untimed steps, then K timed steps bracketed by device syncs, with the
+    per-stage HIP events on; returns (seconds, stage stats)."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    S.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = S.profile_stats()
+    S.profile(False)
+    return dt, stats
+
+
+def show(name, stats, ms_per_step, value):
+    log("%s: %.1f ms/step, %.3f TQU round trips/s" % (name, ms_per_step, value))
+    for st in sorted(stats, key=lambda x: -x["ms"]):
+        log("  %-24s %9.2f ms  %4d launches  %8.2f TFLOP/s  %8.1f GB/s" % (
+            st["name"], st["ms"], st["launches"],
+            st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0,
+            st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0))
+
+
+def contraction_share(name, nt):
+    """Share of the contraction in libsharp's per-triple flop model
+    (libsharp/sharp_core_inc2.c:171-173,440-442,583-585)."""
+    if "spin0" in name:
+        return 4.0 * nt / (4 + 4.0 * nt)
+    if "deriv1" in name:
+        return 8.0 * nt / (12 + 8.0 * nt)
+    return 16.0 * nt / (12 + 16.0 * nt)
+
+
+def roofline(stats, mixed, nt):
+    """Roofline object of the slowest stage.  Legendre stages carry libsharp's
+    flop model per launch (c0 + c1*ntrans per triple: c0 the FP64 recurrence,
+    c1*ntrans the contraction); ring stages carry compulsory HBM bytes."""
+    dom = max(stats, key=lambda x: x["ms"])
+    avg_s = dom["ms"] * 1e-3 / dom["launches"]
+    if dom["flops"] > 0:
+        ach = dom["flops"] / dom["launches"] / avg_s
+        if mixed:
+            c = contraction_share(dom["name"], nt)
+            ach_c = ach * c
+            return {"kernel": dom["name"], "bound": "mfma", "achieved": ach_c / 1e12,
+                    "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+                    "frac": ach_c / PEAK["fp32"], "traffic": None,
+                    "avg_launch_ms": avg_s * 1e3,
+                    "fp64_recurrence_tflops": ach * (1 - c) / 1e12,
+                    "note": "achieved = contraction flops of libsharp's flop model "
+                            "(c1*ntrans per triple) over the FP32 MFMA peak; the FP64 "
+                            "recurrence share (c0 per triple) is given separately"}
+        return {"kernel": dom["name"], "bound": "mfma", "achieved": ach / 1e12,
+                "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                "frac": ach / PEAK["fp64"], "traffic": None,
+                "avg_launch_ms": avg_s * 1e3,
+                "note": "FP64 work priced at the MI355X FP64 peak (vector and matrix "
+                        "FP64 peaks are equal); flops = libsharp's flop model"}
+    ach = dom["bytes"] / dom["launches"] / avg_s
+    return {"kernel": dom["name"], "bound": "hbm", "achieved": ach / 1e9,
+            "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / PEAK["hbm"],
+            "traffic": None, "avg_launch_ms": avg_s * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +201,11 @@ def main():
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=None)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--form", default="fp64", choices=["fp64"])
+    ap.add_argument("--form", default="mixed", choices=["mixed", "fp64"],
+                    help="form whose throughput is `value`; the other one is "
+                         "timed beside it")
+    ap.add_argument("--only", action="store_true",
+                    help="time only --form (no comparison run)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -156,23 +228,12 @@ def main():
     npix = 12 * nside * nside
     geom = S.Geometry.healpix(nside)
     ainfo = S.AlmInfo.triangular(lmax)
-    flags = S.DP
 
     log("generating inputs: nside %d lmax %d batch %d" % (nside, lmax, batch))
     T, E, B = make_alms(torch, lmax, batch, args.seed, dev)
     EB = [x for pair in zip(E, B) for x in pair]
     zeros = lambda: torch.zeros(npix, dtype=torch.float64, device=dev)  # noqa: E731
-    map_t = [zeros() for _ in range(batch)]
-    map_qu = [zeros() for _ in range(2 * batch)]
-    out_t = [torch.zeros_like(T[0]) for _ in range(batch)]
-    out_eb = [torch.zeros_like(T[0]) for _ in range(2 * batch)]
     stream = torch.cuda.current_stream()
-
-    def step():
-        S.execute_device(S.ALM2MAP, 0, T, map_t, geom, ainfo, batch, flags, stream)
-        S.execute_device(S.ALM2MAP, 2, EB, map_qu, geom, ainfo, batch, flags, stream)
-        S.execute_device(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags, stream)
-        S.execute_device(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags, stream)
 
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).normal_()
@@ -181,84 +242,97 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 1 GiB read + 1 GiB written")
         del src, dst
-    log("warmup x%d" % args.warmup)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log("timed x%d" % args.steps)
-    S.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    stats = S.profile_stats()
-    S.profile(False)
-    ms_per_step = 1e3 * dt / args.steps
-    value = batch * args.steps / dt
-    log("%.1f ms/step, %.3f TQU round trips/s" % (ms_per_step, value))
-    for st in sorted(stats, key=lambda x: -x["ms"]):
-        log("  %-24s %9.2f ms  %4d launches  %8.2f TFLOP/s  %8.1f GB/s" % (
-            st["name"], st["ms"], st["launches"],
-            st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0,
-            st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0))
 
-    dom = max(stats, key=lambda x: x["ms"])
-    avg_s = dom["ms"] * 1e-3 / dom["launches"]
-    if dom["flops"] > 0:
-        ach = dom["flops"] / dom["launches"] / avg_s
-        roof = {"kernel": dom["name"], "bound": "mfma", "achieved": ach / 1e12,
-                "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-                "frac": ach / PEAK["fp64"], "traffic": None,
-                "avg_launch_ms": avg_s * 1e3,
-                "note": "FP64 work priced at the MI355X FP64 peak (vector and "
-                        "matrix FP64 peaks are equal); flops = libsharp's flop "
-                        "model per launch"}
-    else:
-        ach = dom["bytes"] / dom["launches"] / avg_s
-        roof = {"kernel": dom["name"], "bound": "hbm", "achieved": ach / 1e9,
-                "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / PEAK["hbm"],
-                "traffic": None, "avg_launch_ms": avg_s * 1e3}
+    forms = [args.form] if args.only else [args.form, "fp64" if args.form == "mixed" else "mixed"]
+    res = {}
+    for form in forms:
+        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+        map_t = [zeros() for _ in range(batch)]
+        map_qu = [zeros() for _ in range(2 * batch)]
+        out_t = [torch.zeros_like(T[0]) for _ in range(batch)]
+        out_eb = [torch.zeros_like(T[0]) for _ in range(2 * batch)]
+
+        def step():
+            S.execute_device(S.ALM2MAP, 0, T, map_t, geom, ainfo, batch, flags, stream)
+            S.execute_device(S.ALM2MAP, 2, EB, map_qu, geom, ainfo, batch, flags, stream)
+            S.execute_device(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags, stream)
+            S.execute_device(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags, stream)
+
+        log("%s: warmup x%d, timed x%d" % (form, args.warmup, args.steps))
+        dt, stats = run_form(S, torch, step, args.warmup, args.steps)
+        ms = 1e3 * dt / args.steps
+        val = batch * args.steps / dt
+        show(form, stats, ms, val)
+        res[form] = {"value": val, "ms_per_step": ms, "stats": stats,
+                     "roofline": roofline(stats, form == "mixed", batch),
+                     "maps": map_t + map_qu, "alms": out_t + out_eb}
+
+    # mixed vs FP64 on the GPU: maps after alm2map, a_lm after the round trip
+    mix_err = None
+    if "mixed" in res and "fp64" in res:
+        def rel(x, y):
+            return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+        mm, fm = res["mixed"]["maps"], res["fp64"]["maps"]
+        ma, fa = res["mixed"]["alms"], res["fp64"]["alms"]
+        mix_err = {
+            "maps_T_max_rel_l2": max(rel(mm[i], fm[i]) for i in range(batch)),
+            "maps_QU_max_rel_l2": max(rel(mm[batch + i], fm[batch + i])
+                                      for i in range(2 * batch)),
+            "roundtrip_alm_T_max_rel_l2": max(rel(ma[i], fa[i]) for i in range(batch)),
+            "roundtrip_alm_EB_max_rel_l2": max(rel(ma[batch + i], fa[batch + i])
+                                               for i in range(2 * batch)),
+        }
+        mix_err["max"] = max(mix_err.values())
+        mix_err["bound"] = 1e-6
+        log("mixed vs FP64:", mix_err)
 
     cpu = None
     parity = None
-    if not args.no_cpu:
+    if not args.no_cpu and "fp64" in res:
         log("cpu baseline (reference libsharp, one realisation)")
         T0 = T[0].cpu().numpy()
         E0 = E[0].cpu().numpy()
         B0 = B[0].cpu().numpy()
         cpu, rmaps, ralms = cpu_baseline(nside, lmax, T0, E0, B0)
         if cpu is not None:
-            gm = (map_t[0].cpu().numpy(), map_qu[0].cpu().numpy(), map_qu[1].cpu().numpy())
-            ga = (out_t[0].cpu().numpy(), out_eb[0].cpu().numpy(), out_eb[1].cpu().numpy())
+            fm, fa = res["fp64"]["maps"], res["fp64"]["alms"]
+            gm = (fm[0].cpu().numpy(), fm[batch].cpu().numpy(), fm[batch + 1].cpu().numpy())
+            ga = (fa[0].cpu().numpy(), fa[batch].cpu().numpy(), fa[batch + 1].cpu().numpy())
             rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))  # noqa: E731
             parity = {"maps_TQU_rel_l2": [rel(x, y) for x, y in zip(gm, rmaps)],
                       "alms_TEB_rel_l2": [rel(x, y) for x, y in zip(ga, ralms)]}
-            log("parity vs reference:", parity)
+            log("FP64 form vs reference:", parity)
 
+    main_form = res[args.form]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_form["value"],
         "unit": "TQU alm2map+map2alm round trips/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_form["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-mfma contraction, f64 recurrence/FFT" if args.form == "mixed" else "f64",
         "data": "synthetic: seeded Gaussian a_lm, C_l^TT=1/(l(l+1)+1) x 5' beam, "
                 "EE=0.1TT, BB=0.01TT, TE corr 0.5",
         "config": {"workload": "HEALPix Nside %d, lmax %d, TQU, batch %d, "
                                "alm2map+map2alm" % (nside, lmax, batch),
                    "nside": nside, "lmax": lmax, "batch": batch, "form": args.form},
-        "roofline": roof,
+        "roofline": main_form["roofline"],
         "cpu_baseline": cpu,
     }
+    for form in res:
+        if form != args.form:
+            out[form + "_form"] = {"value": res[form]["value"],
+                                   "ms_per_step": res[form]["ms_per_step"],
+                                   "roofline": res[form]["roofline"]}
+    if mix_err is not None:
+        out["mixed_vs_fp64"] = mix_err
     if parity is not None:
-        out["parity_vs_reference"] = parity
+        out["fp64_vs_reference"] = parity
     print(json.dumps(out), flush=True)
 
 

@@ -97,6 +97,9 @@ struct GeomPlan {
     int lmax, spin;
     std::vector<int> h;
     int *d = nullptr;
+    // largest mlim of each group of kMixGroupSlots slots (mixed analysis)
+    std::vector<int> hgmax;
+    int *gmax = nullptr;
   };
   std::vector<Mlim> mlims;
   // ring transforms: FFT plans grouped by LDS class, the rings too long for
@@ -109,7 +112,7 @@ struct GeomPlan {
   ~GeomPlan()
   {
     (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
-    for (auto &m : mlims) (void)hipFree(m.d);
+    for (auto &m : mlims) { (void)hipFree(m.d); (void)hipFree(m.gmax); }
     for (auto *p : fft) (void)hipFree(p);
     (void)hipFree(dft_list);
     (void)hipFree(all_list);
@@ -203,7 +206,7 @@ struct Ctx {
   std::vector<std::unique_ptr<YlmPlan>> ylms;
   std::vector<std::unique_ptr<NormPlan>> norms;
   std::vector<std::unique_ptr<AlmPlan>> alms;
-  DevBuf phase, almtmp, stage_alm, stage_map;
+  DevBuf phase, almtmp, part, stage_alm, stage_map;
 };
 
 static std::mutex g_mutex;
@@ -377,6 +380,11 @@ static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
   for (int k = 0; k < gp.nslots; ++k)
     ml.h[k] = sharp_get_mlim(lmax, spin, gp.hsth[k], gp.hcth[k]);
   ml.d = upload(ml.h);
+  const int ng = (gp.nslots + kMixGroupSlots - 1) / kMixGroupSlots;
+  ml.hgmax.assign(ng, -1);
+  for (int k = 0; k < gp.nslots; ++k)
+    ml.hgmax[k / kMixGroupSlots] = std::max(ml.hgmax[k / kMixGroupSlots], ml.h[k]);
+  ml.gmax = upload(ml.hgmax);
   gp.mlims.push_back(std::move(ml));
   return gp.mlims.back();
 }
@@ -569,8 +577,8 @@ static Job make_job(int type, int spin, const sharp_geom_info *g,
         SHP_FAIL("bad ring size");
     }
   }
-  if (j.flags & (SHARP_HIP_MIXED_FP32 | SHARP_HIP_MIXED_BF16X3))
-    SHP_FAIL("mixed-precision forms are provided by the batched entry points");
+  if (j.flags & SHARP_HIP_MIXED_BF16X3)
+    SHP_FAIL("SHARP_HIP_MIXED_BF16X3 is not available in this build");
   return j;
 }
 
@@ -623,7 +631,9 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   const int nmb = (j.mmax + MB) / MB;
   const size_t nphase = (size_t)ncomp_map * nmb * gp.nslots * 2 * MB;
   double2 *phase = static_cast<double2 *>(c.phase.get(nphase * sizeof(double2)));
-  const size_t nalmt = (size_t)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm;
+  // kAlmSlackRows rows of slack: the mixed synthesis kernels read up to 15
+  // rows past lmax (masked) without bounds checks
+  const size_t nalmt = ((size_t)tri_ofs(j.mmax + 1, j.lmax) + kAlmSlackRows) * ncomp_alm;
   double2 *almtmp = static_cast<double2 *>(c.almtmp.get(nalmt * sizeof(double2)));
 
   AlmArgs aa{};
@@ -638,6 +648,11 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   aa.norm_l = d_norm;
   aa.almtmp = almtmp;
   for (int i = 0; i < ncomp_alm; ++i) aa.alms.p[i] = dalm[i];
+  aa.part = nullptr;
+  aa.gmax = ml.gmax;
+  aa.ngroups = (int)ml.hgmax.size();
+  aa.c0 = 0;
+  aa.npc = ncomp_alm;
 
   RingArgs ra{};
   ra.nslots = gp.nslots;
@@ -740,14 +755,53 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   };
   const double alm_bytes = (double)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm * (16. + 2. * esz);
 
+  const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
+  if (!mixed) {
+    if (j.type == SHARP_ALM2MAP) {
+      timed(c, st, "alm_gather", 0., alm_bytes, [&] { launch_alm_gather(aa, st); });
+      for_groups(syn_name, [&](int g) { launch_synth(la, g, st); });
+      phase_to_rings();
+    } else {
+      rings_to_phase();
+      for_groups(ana_name, [&](int g) { launch_anal(la, g, st); });
+      timed(c, st, "alm_scatter", 0., alm_bytes, [&] { launch_alm_scatter(aa, st); });
+    }
+    return;
+  }
+
+  // mixed precision: transforms in groups of up to 16 (spin 0) or 8 (spin s)
+  // per launch, i.e. up to 32 real MFMA columns
+  const int tmax = (j.spin == 0) ? kMixMaxT0 : kMixMaxTS;
+  const char *msyn = j.deriv ? "mixed_synth_deriv1"
+                             : (j.spin == 0 ? "mixed_synth_spin0" : "mixed_synth_spin");
+  const char *mana = j.spin == 0 ? "mixed_anal_spin0" : "mixed_anal_spin";
   if (j.type == SHARP_ALM2MAP) {
     timed(c, st, "alm_gather", 0., alm_bytes, [&] { launch_alm_gather(aa, st); });
-    for_groups(syn_name, [&](int g) { launch_synth(la, g, st); });
+    for (int t = 0; t < j.ntrans; t += tmax) {
+      la.t0 = t;
+      la.ntl = std::min(tmax, j.ntrans - t);
+      timed(c, st, msyn, triples * (c0 + c1 * la.ntl), 0., [&] { launch_msynth(la, st); });
+    }
     phase_to_rings();
   } else {
     rings_to_phase();
-    for_groups(ana_name, [&](int g) { launch_anal(la, g, st); });
-    timed(c, st, "alm_scatter", 0., alm_bytes, [&] { launch_alm_scatter(aa, st); });
+    const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
+    const int ng = (int)ml.hgmax.size();
+    for (int t = 0; t < j.ntrans; t += tmax) {
+      la.t0 = t;
+      la.ntl = std::min(tmax, j.ntrans - t);
+      const int npc = la.ntl * j.nalm;
+      la.ngroups = ng;
+      la.gmax = ml.gmax;
+      la.part = static_cast<float *>(
+          c.part.get((size_t)ng * nrows * 2 * npc * sizeof(float)));
+      timed(c, st, mana, triples * (c0 + c1 * la.ntl), 0., [&] { launch_manal(la, st); });
+      aa.part = la.part;
+      aa.c0 = t * j.nalm;
+      aa.npc = npc;
+      timed(c, st, "alm_scatter", 0., alm_bytes * npc / ncomp_alm,
+            [&] { launch_alm_scatter(aa, st); });
+    }
   }
 }
 
@@ -903,6 +957,7 @@ void sharp_hip_release(void)
   c.alms.clear();
   c.phase.release();
   c.almtmp.release();
+  c.part.release();
   c.stage_alm.release();
   c.stage_map.release();
 }

@@ -24,11 +24,25 @@ struct LegArgs {
   const double2 *alm;   // synthesis input  (canonical almtmp)
   double2 *almout;      // analysis output  (canonical almtmp)
   double2 *phase;
+  // mixed-precision (FP32 MFMA) forms only
+  int ntl;              // transforms in this launch
+  int ngroups;          // analysis: ring-pair groups per m (partial sums)
+  const int *gmax;      // analysis: largest mlim of each group [ngroups]
+  float *part;          // analysis: partial a_lm [ngroups][rows][2*ntl*nalm]
 };
 
 // ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st);
 void launch_anal(const LegArgs &a, int ntg, hipStream_t st);
+
+// Mixed-precision forms: FP64 recurrence, FP32 MFMA contraction over a batch
+// of up to kMixMaxT transforms per launch (a.ntl).
+constexpr int kMixMaxT0 = 16;   // spin 0: 2 real columns per transform
+constexpr int kMixMaxTS = 8;    // spin s: 4 real columns per transform
+constexpr int kMixWaves = 8;    // analysis: wavefronts per workgroup
+constexpr int kMixGroupSlots = 64 * kMixWaves;
+void launch_msynth(const LegArgs &a, hipStream_t st);
+void launch_manal(const LegArgs &a, hipStream_t st);
 
 // Arguments of the ring stage (phase <-> map).
 constexpr int kMaxComp = 200;   // ntrans <= 100, two components each
@@ -84,6 +98,12 @@ struct AlmArgs {
   const double *norm_l; // [lmax+1]
   double2 *almtmp;
   PtrTab alms;          // [ncomp] device pointers (float or double data)
+  // analysis from partial sums (mixed forms): when part != nullptr the
+  // components [c0, c0+npc) are sum_g part[g] over the groups g with
+  // gmax[g] >= m, instead of almtmp
+  const float *part;
+  const int *gmax;
+  int ngroups, c0, npc;
 };
 void launch_alm_gather(const AlmArgs &a, hipStream_t st);
 void launch_alm_scatter(const AlmArgs &a, hipStream_t st);

@@ -1,0 +1,121 @@
+// legendre_common.hpp -- device helpers shared by the FP64 and the mixed-
+// precision Legendre kernels: exponent-scaled start values and the spin-0 /
+// spin-s recurrence steps, in libsharp's operation order
+// (libsharp/sharp_core_inc.c:73-293).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "shp_internal.hpp"
+
+namespace shp {
+
+// Brings |val| into [maxval*2^-800, maxval].  A finite double needs at most
+// three 800-binade steps; the bound also ends the loop for non-finite input.
+__device__ inline void normalize(double &val, int &scale, double maxval)
+{
+  for (int it = 0; it < 4 && fabs(val) > maxval; ++it) { val *= kFSmall; ++scale; }
+  if (val != 0.)
+    for (int it = 0; it < 4 && fabs(val) < maxval * kFSmall; ++it) {
+      val *= kFBig;
+      --scale;
+    }
+}
+
+// val^npow with a separate exponent (libsharp/sharp_core_inc.c:97-126)
+__device__ inline void mypow(double val, int npow, double &resd, int &ress)
+{
+  int scale = 0, scaleint = 0;
+  double res = 1.;
+  normalize(val, scaleint, kFBigHalf);
+  do {
+    if (npow & 1) {
+      res *= val;
+      scale += scaleint;
+      normalize(res, scale, kFBigHalf);
+    }
+    val *= val;
+    scaleint += scaleint;
+    normalize(val, scaleint, kFBigHalf);
+  } while (npow >>= 1);
+  resd = res;
+  ress = scale;
+}
+
+__device__ inline double corfac(int scale)
+{
+  return (scale < 0) ? 0. : ((scale == 0) ? 1. : kFBig);
+}
+
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
+__device__ inline bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+
+struct SpinStart {
+  int mhi, cosPow, sinPow, preMinus_p, preMinus_m;
+};
+
+__device__ inline SpinStart spin_start(int m, int s)
+{
+  SpinStart r;
+  const int mhi = (m < s) ? s : m;
+  r.mhi = mhi;
+  r.preMinus_p = r.preMinus_m = 0;
+  if (mhi == m) {
+    r.cosPow = mhi + s;
+    r.sinPow = mhi - s;
+    r.preMinus_p = r.preMinus_m = ((mhi - s) & 1);
+  } else {
+    r.cosPow = mhi + m;
+    r.sinPow = mhi - m;
+    r.preMinus_m = ((mhi + m) & 1);
+  }
+  return r;
+}
+
+// start values of lambda_plus / lambda_minus at l=mhi with their scales
+// (libsharp/sharp_core_inc.c:223-273)
+__device__ inline void spin_init(double cth, double sth, int s, const SpinStart &st,
+                                 double prefac, int fscale, double &rec2p,
+                                 double &rec2m, int &scalep, int &scalem)
+{
+  double cth2 = fmax(sqrt((1. + cth) * 0.5), 1e-15);
+  double sth2 = fmax(sqrt((1. - cth) * 0.5), 1e-15);
+  if (sth < 0. && cth < 0.) cth2 = -cth2;
+  if (sth < 0. && cth > 0.) sth2 = -sth2;
+  double ccp, ssp, csp, scp;
+  int ccps, ssps, csps, scps;
+  mypow(cth2, st.cosPow, ccp, ccps);
+  mypow(sth2, st.sinPow, ssp, ssps);
+  mypow(cth2, st.sinPow, csp, csps);
+  mypow(sth2, st.cosPow, scp, scps);
+  rec2p = prefac * ccp; scalep = fscale + ccps;
+  rec2m = prefac * csp; scalem = fscale + csps;
+  normalize(rec2m, scalem, kFBigHalf);
+  normalize(rec2p, scalep, kFBigHalf);
+  rec2p *= ssp; scalep += ssps;
+  rec2m *= scp; scalem += scps;
+  if (st.preMinus_p) rec2p = -rec2p;
+  if (st.preMinus_m) rec2m = -rec2m;
+  if (s & 1) rec2p = -rec2p;
+  normalize(rec2m, scalem, kFTol);
+  normalize(rec2p, scalep, kFTol);
+}
+
+struct Fx { double f0, f1, f2; };
+
+__device__ inline void rec_step(double &rxp, double &rxm, double ryp, double rym,
+                                double cth, const Fx &f)
+{
+  rxp = ((cth - f.f1) * (f.f0 * ryp)) - f.f2 * rxp;
+  rxm = ((cth + f.f1) * (f.f0 * rym)) - f.f2 * rxm;
+}
+
+__device__ inline bool rescale1(double &l1, double &l2, int &scale)
+{
+  if (fabs(l2) > kFTol) {
+    l1 *= kFSmall; l2 *= kFSmall; ++scale;
+    return true;
+  }
+  return false;
+}
+
+} // namespace shp

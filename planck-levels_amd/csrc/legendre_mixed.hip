@@ -1,0 +1,1050 @@
+// legendre_mixed.hip -- mixed-precision Legendre stage on CDNA4 (gfx950).
+//
+// The lambda_lm recurrence, its start values and its exponent scaling run in
+// FP64 exactly as in the FP64 kernels (same operations, same order, same
+// switch points between the scaled and the unscaled domain).  Only the
+// contraction of lambda with the batch of transforms runs on the FP32 matrix
+// cores, v_mfma_f32_16x16x4f32:  C[16x16] += A[16x4] . B[4x16], where lane
+// i+16k holds A[i][k] and B[k][i], and lane n+16q holds C[4q..4q+3][n].
+//
+// Synthesis: one wavefront per (m, 64 ring pairs), lane = ring pair.  A lane
+// produces lambda for 8 consecutive l (spin 0) or lambda_w / lambda_x for 4
+// consecutive l (spin s); per accumulator that is one K=4 slice of A with the
+// ring pair in the lane and the l index in the register.  Two permlane32 and
+// two permlane16 swaps per 4 registers exchange the 16-lane rows so that the
+// register index becomes the lane row, which is A's K layout.  B is the
+// [4 l x 16 columns] slice of the a_lm, read from the canonical FP64 rows and
+// rounded to FP32.  The N/S parity sums p1/p2 of the FP64 kernel are separate
+// MFMA accumulators; for spin s the lambda_x terms go to the accumulator of
+// the other parity with the rotated a_lm columns (libsharp/sharp_core_inc2.c:
+// 282-306).  The MFMA accumulators are emptied into FP32 totals every 8 K
+// slices, which keeps the rounding error of long sums near unit roundoff.
+//
+// Analysis: a workgroup of kMixWaves wavefronts per (m, group of 64*kMixWaves
+// ring pairs).  Each lane runs the recurrence for its ring pair 32 l at a time
+// and stores lambda (FP32) in its wavefront's LDS slice; the MFMA reads it back
+// transposed, A = [16 l of one parity x 4 ring pairs], against
+// B = [4 ring pairs x 16 columns], the phases of the wavefront's 64 ring pairs,
+// which stay in registers for the whole l range.  The wavefronts' [16 l x 16]
+// tiles are added in LDS in a fixed order and stored as the group's partial
+// a_lm (FP32); the a_lm scatter adds the groups in FP64 in a fixed order, so
+// results do not depend on scheduling.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2;
+// xcd_map keeps all workgroups of one m (analysis) or m-block (synthesis) on
+// one XCD, so the recurrence tables and a_lm rows of that m are fetched into
+// one L2 only.
+
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+#include "legendre_common.hpp"
+
+namespace shp {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f4 mfma4(float a, float b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// On entry v[k] of lane 16r+i belongs to lane row r; on exit v[r] of lane
+// 16k+i holds what was v[k] of lane 16r+i (a 4x4 transpose of lane rows
+// against registers).
+__device__ inline void row_transpose4(float v[4])
+{
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[0]),
+                                                  __float_as_uint(v[2]), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[1]),
+                                                  __float_as_uint(v[3]), false, false);
+  const auto c = __builtin_amdgcn_permlane16_swap(a[0], b[0], false, false);
+  const auto d = __builtin_amdgcn_permlane16_swap(a[1], b[1], false, false);
+  v[0] = __uint_as_float(c[0]);
+  v[1] = __uint_as_float(c[1]);
+  v[2] = __uint_as_float(d[0]);
+  v[3] = __uint_as_float(d[1]);
+}
+
+// Maps a linear workgroup id to (major, minor) so that the nminor workgroups
+// of one major index land on the same XCD (ids are dealt round-robin over 8).
+// False for the padding ids of the last round.
+__device__ inline bool xcd_map(int id, int nmajor, int nminor, int &major, int &minor)
+{
+  const int xcd = id & 7, k = id >> 3;
+  major = (k / nminor) * 8 + xcd;
+  minor = k % nminor;
+  return major < nmajor;
+}
+
+static inline int xcd_grid(int nmajor, int nminor)
+{
+  return ((nmajor + 7) / 8) * 8 * nminor;
+}
+
+constexpr int kFlushBlocks = 8;   // K slices per FP32 partial sum
+
+template <int NCB>
+__device__ inline void flush(f4 (&tot)[NCB][4], f4 (&c)[NCB][4])
+{
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      tot[cb][r] += c[cb][r];
+      c[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
+// stores N = t1+t2 and S = sg*(t1-t2) of the 64 ring pairs of a wavefront:
+// lane n+16q holds rows 16r+4q+v (v = 0..3) of column n of accumulator r
+template <int NCB>
+__device__ inline void store_phases(const LegArgs &a, int m, int sb, int comp0, int ncol,
+                                    double sg, const f4 (&t1)[NCB][4],
+                                    const f4 (&t2)[NCB][4])
+{
+  const int lane = threadIdx.x, n = lane & 15, q = lane >> 4;
+  double *ph = reinterpret_cast<double *>(a.phase);
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb) {
+    const int col = cb * 16 + n;
+    if (col >= ncol) continue;
+    const int comp = comp0 + (col >> 1), part = col & 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ring = sb * 64 + 16 * r + 4 * q + v;
+        if (ring >= a.nslots) continue;
+        const double x1 = t1[cb][r][v], x2 = t2[cb][r][v];
+        ph[2 * phase_index(comp, m, ring, 0, a.nslots, a.nmb) + part] = x1 + x2;
+        if (a.ispair[ring])
+          ph[2 * phase_index(comp, m, ring, 1, a.nslots, a.nmb) + part] = sg * (x1 - x2);
+      }
+  }
+}
+
+// ===========================================================================
+// spin 0 synthesis
+// ===========================================================================
+// Memory operations are issued one block ahead of their use: the a_lm rows
+// of the next block (vector loads, which complete in order) at the top of a
+// block, the recurrence coefficients of the next block (scalar loads, which
+// complete out of order and are waited for as a whole) after the first
+// two-step of a block, so that the wait for the current coefficients does
+// not also wait for the prefetch.
+template <int NCB>
+__global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int m = mb * MB + w;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax, nc = a.ncomp_alm, ncol = 2 * a.ntl;
+  const int slot = sb * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+  const int n = lane & 15, kq = lane >> 4;
+
+  f4 tot1[NCB][4], tot2[NCB][4];
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tot1[cb][r] = tot2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
+
+  if (wave_any(active)) {
+    const double2 *rf =
+        reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+    const int lpad = lmax + kTabPad;   // last readable entry of the row
+    // row l of this m, first column of this launch
+    const double *arow = reinterpret_cast<const double *>(a.alm) +
+                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0) * 2;
+    int colo[NCB];
+    bool colok[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n;
+      colok[cb] = col < ncol;
+      colo[cb] = colok[cb] ? col : 0;
+    }
+    // B rows l0+2k (accumulator 1) and l0+2k+1 (accumulator 2).  Rows past
+    // lmax (up to lmax+15) are read from the next m, or from the slack rows
+    // at the end of the canonical array, and masked where they are used.
+    int boff1[NCB], boff2[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      boff1[cb] = 2 * kq * 2 * nc + colo[cb];
+      boff2[cb] = boff1[cb] + 2 * nc;
+    }
+    auto ldB = [&](int l0, double (&b1)[NCB], double (&b2)[NCB]) {
+      const double *row = arow + (int64_t)l0 * 2 * nc;
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        b1[cb] = row[boff1[cb]];
+        b2[cb] = row[boff2[cb]];
+      }
+    };
+
+    double lam1 = 0., lam2;
+    int scale;
+    mypow(sth, m, lam2, scale);
+    const double mf = a.mfac[m];
+    lam2 *= (m & 1) ? -mf : mf;
+    normalize(lam2, scale, kFTol);
+    // a ring pair beyond its m limit contributes nothing: zero lambda, in
+    // range, so it neither contributes nor holds back the mode switches
+    if (!active) { lam2 = 0.; scale = 0; }
+
+    int l = m;
+    bool dead = false;
+    while (wave_all(scale < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      const double2 r0 = rf[l], r1 = rf[l + 1];
+      lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+      lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+      if (fabs(lam2) > kFTol) { lam1 *= kFSmall; lam2 *= kFSmall; ++scale; }
+      l += 2;
+    }
+    if (!dead) {
+      double cf = corfac(scale);
+      bool scaled = wave_any(scale < 0);
+      if (!scaled) { lam1 *= cf; lam2 *= cf; cf = 1.; }
+      f4 c1[NCB][4], c2[NCB][4];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c1[cb][r] = c2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
+      double2 rc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
+      double bc1[NCB], bc2[NCB];
+      ldB(l, bc1, bc2);
+      auto mfmas = [&](const float (&v1)[4], const float (&v2)[4], const float (&b1)[NCB],
+                       const float (&b2)[NCB]) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            c1[cb][r] = mfma4(v1[r], b1[cb], c1[cb][r]);
+            c2[cb][r] = mfma4(v2[r], b2[cb], c2[cb][r]);
+          }
+      };
+      int nblk = 0;
+      // one block of 8 l with every check (scaled domain, rows past lmax)
+      auto general_block = [&]() {
+        const int lb = l;
+        double bn1[NCB], bn2[NCB];
+        ldB(lb + 8, bn1, bn2);
+        double2 rn[8];
+        float v1[4], v2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double o0 = 0., o1 = 0.;
+          if (l <= lmax) {
+            const double2 r0 = rc[2 * q], r1 = rc[2 * q + 1];
+            o0 = lam2 * cf;
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            o1 = (l + 1 <= lmax) ? lam1 * cf : 0.;
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (scaled && fabs(lam2) > kFTol) {
+              lam1 *= kFSmall; lam2 *= kFSmall; ++scale;
+              cf = corfac(scale);
+            }
+          }
+          v1[q] = (float)o0;
+          v2[q] = (float)o1;
+          l += 2;
+          if (q == 0) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) rn[k] = rf[min(lb + 8 + k, lpad)];
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        row_transpose4(v1);
+        row_transpose4(v2);
+        float b1[NCB], b2[NCB];
+        {
+          const int l1 = lb + 2 * kq, l2 = l1 + 1;
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            b1[cb] = (colok[cb] && l1 <= lmax) ? (float)bc1[cb] : 0.f;
+            b2[cb] = (colok[cb] && l2 <= lmax) ? (float)bc2[cb] : 0.f;
+          }
+        }
+        mfmas(v1, v2, b1, b2);
+        if (scaled && wave_all(scale >= 0)) {
+          lam1 *= cf; lam2 *= cf; cf = 1.;
+          scaled = false;
+        }
+        if (++nblk == kFlushBlocks) {
+          nblk = 0;
+          flush(tot1, c1);
+          flush(tot2, c2);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rc[k] = rn[k];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) { bc1[cb] = bn1[cb]; bc2[cb] = bn2[cb]; }
+      };
+
+      // 1: scaled domain
+      while (scaled && l <= lmax) general_block();
+      // 2: unscaled, whole groups of kFlushBlocks blocks, software-pipelined
+      // (the matrix products of a block are issued during the recurrence of
+      // the next one)
+      constexpr int GL = 8 * kFlushBlocks;   // l per group
+      if (!scaled && l + GL - 1 <= lmax) {
+        float pv1[4] = {0.f, 0.f, 0.f, 0.f}, pv2[4] = {0.f, 0.f, 0.f, 0.f};
+        float pb1[NCB], pb2[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) pb1[cb] = pb2[cb] = 0.f;
+        while (l + GL - 1 <= lmax) {
+#pragma unroll
+          for (int k = 0; k < kFlushBlocks; ++k) {
+            const int lb = l;
+            double bn1[NCB], bn2[NCB];
+            ldB(lb + 8, bn1, bn2);
+            double2 rn[8];
+            float v1[4], v2[4];
+            // first use of this block's coefficients (waits for the scalar
+            // loads in flight), then the next block's requests
+            const double t00 = cth * lam2, r00x = rc[0].x;
+            __builtin_amdgcn_sched_barrier(0);
+            // lb + 15 <= lmax + kTabPad in this phase
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) rn[kk] = rf[lb + 8 + kk];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const double2 r0 = rc[2 * q], r1 = rc[2 * q + 1];
+              v1[q] = (float)lam2;
+              lam1 = (q == 0 ? t00 * r00x : (cth * lam2) * r0.x) - lam1 * r0.y;
+              v2[q] = (float)lam1;
+              lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+              if (q == 1) mfmas(pv1, pv2, pb1, pb2);
+            }
+            row_transpose4(v1);
+            row_transpose4(v2);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+              pb1[cb] = colok[cb] ? (float)bc1[cb] : 0.f;
+              pb2[cb] = colok[cb] ? (float)bc2[cb] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { pv1[q] = v1[q]; pv2[q] = v2[q]; }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) rc[kk] = rn[kk];
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) { bc1[cb] = bn1[cb]; bc2[cb] = bn2[cb]; }
+            l += 8;
+          }
+          flush(tot1, c1);
+          flush(tot2, c2);
+        }
+        mfmas(pv1, pv2, pb1, pb2);
+        nblk = 0;
+      }
+      // 3: the remaining rows
+      while (l <= lmax) general_block();
+      flush(tot1, c1);
+      flush(tot2, c2);
+    }
+  }
+  store_phases<NCB>(a, m, sb, a.t0, ncol, 1., tot1, tot2);
+}
+
+// ===========================================================================
+// spin s synthesis (and ALM2MAP_DERIV1)
+// ===========================================================================
+template <int NCB, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int m = mb * MB + w;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax, nc = a.ncomp_alm, sp = a.spin, ncol = 4 * a.ntl;
+  const int slot = sb * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+  const SpinStart st = spin_start(m, sp);
+  const int n = lane & 15, kq = lane >> 4;
+
+  f4 tot1[NCB][4], tot2[NCB][4];
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tot1[cb][r] = tot2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
+
+  if (wave_any(active) && st.mhi <= lmax) {
+    const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+    const int lpad = lmax + kTabPad;
+    auto fx = [&](int l) {
+      const int k = min(l, lpad);
+      return Fx{fxb[3 * k], fxb[3 * k + 1], fxb[3 * k + 2]};
+    };
+    // unclamped: for l <= lpad only
+    auto fxu = [&](int l) { return Fx{fxb[3 * l], fxb[3 * l + 1], fxb[3 * l + 2]}; };
+    constexpr int NALM = DERIV ? 1 : 2;
+    const double *arow = reinterpret_cast<const double *>(a.alm) +
+                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0 * NALM) * 2;
+    // Column col = 4t+q is (Q.re, Q.im, U.re, U.im)[q] of transform t.  Its
+    // B entry is X = (E.re, E.im, B.re, B.im)[q] for lambda_w and
+    // Y = (-B.im, B.re, E.im, -E.re)[q] for lambda_x (libsharp/
+    // sharp_core_inc2.c:282-306): offsets into the row and signs (offset 0
+    // and sign 0 for unused columns).
+    int xo[NCB], yo[NCB];
+    float xs[NCB], ys[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n, t = col >> 2, q = col & 3;
+      const bool ok = col < ncol;
+      if (!DERIV) {
+        xo[cb] = ok ? col : 0;
+        xs[cb] = ok ? 1.f : 0.f;
+        yo[cb] = ok ? (col ^ 3) : 0;
+        ys[cb] = ok ? ((q == 0 || q == 3) ? -1.f : 1.f) : 0.f;
+      } else {
+        // no curl a_lm: X = (E.re, E.im, 0, 0), Y = (0, 0, E.im, -E.re)
+        xo[cb] = ok ? 2 * t + (q & 1) : 0;
+        xs[cb] = (ok && q < 2) ? 1.f : 0.f;
+        yo[cb] = ok ? 2 * t + (q == 2 ? 1 : 0) : 0;
+        ys[cb] = ok ? (q == 2 ? 1.f : (q == 3 ? -1.f : 0.f)) : 0.f;
+      }
+    }
+    // B row l0+k.  Rows past lmax (up to lmax+7) are read from the next m,
+    // or from the slack rows at the end of the canonical array, and masked
+    // where they are used.
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      xo[cb] += kq * 2 * nc;
+      yo[cb] += kq * 2 * nc;
+    }
+    auto ldB = [&](int l0, double (&bx)[NCB], double (&by)[NCB]) {
+      const double *row = arow + (int64_t)l0 * 2 * nc;
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        bx[cb] = row[xo[cb]];
+        by[cb] = row[yo[cb]];
+      }
+    };
+
+    double rec1p = 0., rec1m = 0., rec2p, rec2m;
+    int scalep, scalem;
+    spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep, scalem);
+    // a ring pair beyond its m limit contributes nothing: zero lambda, in
+    // range, so it neither contributes nor holds back the mode switches
+    if (!active) { rec2p = rec2m = 0.; scalep = scalem = 0; }
+    int l = st.mhi;
+    bool dead = false;
+    while (wave_all(scalep < 1 && scalem < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+      rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+      rescale1(rec1p, rec2p, scalep);
+      rescale1(rec1m, rec2m, scalem);
+      l += 2;
+    }
+    if (!dead) {
+      double cfp = corfac(scalep), cfm = corfac(scalem);
+      bool scaled = wave_any(scalep < 0 || scalem < 0);
+      if (!scaled) {
+        rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+        cfp = cfm = 1.;
+      }
+      f4 c1[NCB][4], c2[NCB][4];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c1[cb][r] = c2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
+      Fx fc[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
+      double bcx[NCB], bcy[NCB];
+      ldB(l, bcx, bcy);
+      const bool odd = (kq & 1) != 0;
+      auto mfmas = [&](const float (&v1)[4], const float (&v2)[4], const float (&b1)[NCB],
+                       const float (&b2)[NCB]) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            c1[cb][r] = mfma4(v1[r], b1[cb], c1[cb][r]);
+            c2[cb][r] = mfma4(v2[r], b2[cb], c2[cb][r]);
+          }
+      };
+      // accumulator 1 (N+S side): K rows (w_l, x_l+1, w_l+2, x_l+3) with B
+      // rows (X_l, Y_l+1, X_l+2, Y_l+3); accumulator 2 the other way round
+      auto bvals = [&](int lb, float (&b1)[NCB], float (&b2)[NCB]) {
+        const bool okr = lb + kq <= lmax;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+          const float bx = okr ? xs[cb] * (float)bcx[cb] : 0.f;
+          const float by = okr ? ys[cb] * (float)bcy[cb] : 0.f;
+          b1[cb] = odd ? by : bx;
+          b2[cb] = odd ? bx : by;
+        }
+      };
+      int nblk = 0;
+      // one block of 4 l with every check (scaled domain, rows past lmax)
+      auto general_block = [&]() {
+        const int lb = l;
+        double bnx[NCB], bny[NCB];
+        ldB(lb + 4, bnx, bny);
+        Fx fn[4];
+        float v1[4], v2[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double lw0 = 0., lx0 = 0., lw1 = 0., lx1 = 0.;
+          if (l <= lmax) {
+            const double rp0 = rec2p * cfp, rm0 = rec2m * cfm;
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+            const double rp1 = rec1p * cfp, rm1 = rec1m * cfm;
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+            if (scaled) {
+              const bool a1 = rescale1(rec1p, rec2p, scalep);
+              const bool a2 = rescale1(rec1m, rec2m, scalem);
+              if (a1 || a2) { cfp = corfac(scalep); cfm = corfac(scalem); }
+            }
+            lw0 = rp0 + rm0;
+            lx0 = rm0 - rp0;
+            if (l + 1 <= lmax) {
+              lw1 = rp1 + rm1;
+              lx1 = rm1 - rp1;
+            }
+          }
+          v1[2 * h] = (float)lw0;
+          v1[2 * h + 1] = (float)lx1;
+          v2[2 * h] = (float)lx0;
+          v2[2 * h + 1] = (float)lw1;
+          l += 2;
+          if (h == 0) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fn[k] = fx(lb + 5 + k);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        row_transpose4(v1);
+        row_transpose4(v2);
+        float b1[NCB], b2[NCB];
+        bvals(lb, b1, b2);
+        mfmas(v1, v2, b1, b2);
+        if (scaled && wave_all(scalep >= 0 && scalem >= 0)) {
+          rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+          cfp = cfm = 1.;
+          scaled = false;
+        }
+        if (++nblk == kFlushBlocks) {
+          nblk = 0;
+          flush(tot1, c1);
+          flush(tot2, c2);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fc[k] = fn[k];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) { bcx[cb] = bnx[cb]; bcy[cb] = bny[cb]; }
+      };
+
+      // 1: scaled domain
+      while (scaled && l <= lmax) general_block();
+      // 2: unscaled, whole groups of kFlushBlocks blocks, software-pipelined:
+      // the matrix products of a block are issued during the FP64
+      // recurrence of the next one
+      if (!scaled && l + 4 * kFlushBlocks - 1 <= lmax) {
+        float pv1[4] = {0.f, 0.f, 0.f, 0.f}, pv2[4] = {0.f, 0.f, 0.f, 0.f};
+        float pb1[NCB], pb2[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) pb1[cb] = pb2[cb] = 0.f;
+        while (l + 4 * kFlushBlocks - 1 <= lmax) {
+#pragma unroll
+          for (int k = 0; k < kFlushBlocks; ++k) {
+            const int lb = l;
+            double bnx[NCB], bny[NCB];
+            ldB(lb + 4, bnx, bny);
+            Fx fn[4];
+            float v1[4], v2[4];
+            // The first use of this block's coefficients waits for every
+            // scalar load in flight; the next block's are requested right
+            // after it, so they have a whole block to arrive.
+            const double c0m = cth - fc[0].f1, c0p = cth + fc[0].f1;
+            __builtin_amdgcn_sched_barrier(0);
+            // lb + 8 <= lmax + kTabPad in this phase
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fn[q] = fxu(lb + 5 + q);
+            __builtin_amdgcn_sched_barrier(0);
+            {
+              const double rp0 = rec2p, rm0 = rec2m;
+              rec1p = (c0m * (fc[0].f0 * rec2p)) - fc[0].f2 * rec1p;
+              rec1m = (c0p * (fc[0].f0 * rec2m)) - fc[0].f2 * rec1m;
+              rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[1]);
+              v1[0] = (float)(rp0 + rm0);
+              v2[0] = (float)(rm0 - rp0);
+              v2[1] = (float)(rec1p + rec1m);
+              v1[1] = (float)(rec1m - rec1p);
+            }
+            {
+              const double rp0 = rec2p, rm0 = rec2m;
+              rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2]);
+              rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[3]);
+              v1[2] = (float)(rp0 + rm0);
+              v2[2] = (float)(rm0 - rp0);
+              v2[3] = (float)(rec1p + rec1m);
+              v1[3] = (float)(rec1m - rec1p);
+            }
+            mfmas(pv1, pv2, pb1, pb2);
+            row_transpose4(v1);
+            row_transpose4(v2);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+              const float bx = xs[cb] * (float)bcx[cb], by = ys[cb] * (float)bcy[cb];
+              pb1[cb] = odd ? by : bx;
+              pb2[cb] = odd ? bx : by;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { pv1[q] = v1[q]; pv2[q] = v2[q]; fc[q] = fn[q]; }
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) { bcx[cb] = bnx[cb]; bcy[cb] = bny[cb]; }
+            l += 4;
+          }
+          flush(tot1, c1);
+          flush(tot2, c2);
+        }
+        mfmas(pv1, pv2, pb1, pb2);
+        nblk = 0;
+      }
+      // 3: the remaining rows
+      while (l <= lmax) general_block();
+      flush(tot1, c1);
+      flush(tot2, c2);
+    }
+  }
+  const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
+  store_phases<NCB>(a, m, sb, 2 * a.t0, ncol, sg, tot1, tot2);
+}
+
+// ===========================================================================
+// analysis
+// ===========================================================================
+constexpr int kLS = 33;    // LDS row stride (floats) of the lambda slices: odd,
+                           // so row-wise stores and the transposed MFMA loads
+                           // hit distinct banks
+constexpr int kTS = 20;    // LDS row stride of the [16 x 16] reduction tiles
+
+// Adds the wavefronts' tiles (in slice 0 of each wavefront) in a fixed order
+// and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm.
+template <int NT>
+__device__ inline void reduce_store(float *const *slices, int nw, float *prow, int lc,
+                                    int lmax, int ncol)
+{
+  for (int e = threadIdx.x; e < NT * 256; e += blockDim.x) {
+    const int tile = e >> 8, i = (e >> 4) & 15, nn = e & 15;
+    const int off = (tile * 16 + i) * kTS + nn;
+    float sum = 0.f;
+    for (int ww = 0; ww < nw; ++ww) sum += slices[ww][off];
+    const int x = tile & 1, cb = tile >> 1;
+    const int ll = lc + x + 2 * i, col = cb * 16 + nn;
+    if (ll <= lmax && col < ncol) prow[(int64_t)ll * ncol + col] = sum;
+  }
+}
+
+template <int NCB>
+__device__ inline void write_tiles(float *slice, const f4 (&t)[2][NCB])
+{
+  const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        slice[((cb * 2 + x) * 16 + 4 * q + v) * kTS + n] = t[x][cb][v];
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWaves) k_manal_spin0(LegArgs a)
+{
+  constexpr int W = kMixWaves;
+  static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
+  __shared__ float lds[W][64 * kLS];
+  int m, g;
+  if (!xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) return;
+  if (a.gmax[g] < m) return;   // no ring pair of this group reaches m
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lmax = a.lmax, ncol = 2 * a.ntl, nslots = a.nslots;
+  const int wbase = g * 64 * W + w * 64;
+  const int slot = wbase + lane;
+  const bool valid = slot < nslots;
+  const int s = valid ? slot : nslots - 1;
+  const bool active = valid && (a.mlim[s] >= m);
+  const int n = lane & 15, kq = lane >> 4;
+  const double *ph = reinterpret_cast<const double *>(a.phase);
+  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * ncol +
+                (int64_t)(tri_ofs(m, lmax) - m) * ncol;
+
+  // B: p1 = N+S and p2 = N-S of ring pairs wbase+4j+kq, column n
+  float bp[2][16][NCB];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int rs = wbase + 4 * j + kq;
+    const bool ok = rs < nslots && a.mlim[rs < nslots ? rs : 0] >= m;
+    const bool pr = ok && a.ispair[rs] != 0;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n;
+      double nv = 0., sv = 0.;
+      if (ok && col < ncol) {
+        const int comp = a.t0 + (col >> 1), part = col & 1;
+        nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
+        if (pr) sv = ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
+      }
+      bp[0][j][cb] = (float)(nv + sv);
+      bp[1][j][cb] = (float)(nv - sv);
+    }
+  }
+
+  const double cth = a.cth[s], sth = a.sth[s];
+  const double2 *rf = reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+  const int lpad = lmax + kTabPad;
+  double lam1 = 0., lam2;
+  int scale;
+  mypow(sth, m, lam2, scale);
+  const double mf = a.mfac[m];
+  lam2 *= (m & 1) ? -mf : mf;
+  normalize(lam2, scale, kFTol);
+  // 0: advancing below the IEEE range, 1: scaled, 2: unscaled, 3: idle
+  int mode = !wave_any(active) ? 3 : (wave_all(scale < 1) ? 0 : 1);
+  double cf = corfac(scale);
+  int l = m;
+  float *mine = lds[w];
+  float *slices[W];
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww];
+  // recurrence coefficients of the current 8 l; the next 8 are fetched
+  // after the first two-step of each group of 8 (see the synthesis kernels)
+  double2 rc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
+
+  const int nchunks = (lmax - m + 32) / 32;
+  for (int c = 0; c < nchunks; ++c) {
+    const int lc = m + 32 * c;
+    bool any = false;
+#pragma unroll
+    for (int g8 = 0; g8 < 4; ++g8) {
+      const int lb = l;
+      double2 rn[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g8 + q;
+        double o0 = 0., o1 = 0.;
+        if (mode != 3 && l <= lmax) {
+          const double2 r0 = rc[2 * q], r1 = rc[2 * q + 1];
+          if (mode == 0) {
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (fabs(lam2) > kFTol) { lam1 *= kFSmall; lam2 *= kFSmall; ++scale; }
+            if (wave_any(scale >= 1)) { mode = 1; cf = corfac(scale); }
+          } else {
+            any = true;
+            o0 = lam2 * cf;
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            o1 = (l + 1 <= lmax) ? lam1 * cf : 0.;
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (mode == 1) {
+              if (fabs(lam2) > kFTol) {
+                lam1 *= kFSmall; lam2 *= kFSmall; ++scale;
+                cf = corfac(scale);
+              }
+              if (wave_all(scale >= 0)) {
+                lam1 *= cf; lam2 *= cf; cf = 1.;
+                mode = 2;
+              }
+            }
+          }
+        }
+        mine[lane * kLS + 2 * j] = active ? (float)o0 : 0.f;
+        mine[lane * kLS + 2 * j + 1] = active ? (float)o1 : 0.f;
+        l += 2;
+        if (q == 0) {
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) rn[k] = rf[min(lb + 8 + k, lpad)];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) rc[k] = rn[k];
+    }
+    // the slice is read back by the same wavefront: LDS operations of one
+    // wavefront complete in order, the fence keeps the compiler from moving
+    // the loads above the stores
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    f4 t[2][NCB];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) t[x][cb] = f4{0.f, 0.f, 0.f, 0.f};
+    if (any) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const float A = mine[(4 * j + kq) * kLS + x + 2 * n];
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma4(A, bp[x][j][cb], t[x][cb]);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    write_tiles<NCB>(mine, t);
+    __syncthreads();
+    reduce_store<2 * NCB>(slices, W, prow, lc, lmax, ncol);
+    __syncthreads();
+  }
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWaves) k_manal_spin(LegArgs a)
+{
+  constexpr int W = kMixWaves;
+  static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
+  __shared__ float lds[W][2][64 * kLS];
+  int m, g;
+  if (!xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) return;
+  if (a.gmax[g] < m) return;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lmax = a.lmax, ncol = 4 * a.ntl, nslots = a.nslots, sp = a.spin;
+  const SpinStart st = spin_start(m, sp);
+  const int wbase = g * 64 * W + w * 64;
+  const int slot = wbase + lane;
+  const bool valid = slot < nslots;
+  const int s = valid ? slot : nslots - 1;
+  const bool active = valid && (a.mlim[s] >= m);
+  const int n = lane & 15, kq = lane >> 4;
+  const double *ph = reinterpret_cast<const double *>(a.phase);
+  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * ncol +
+                (int64_t)(tri_ofs(m, lmax) - m) * ncol;
+
+  // rows m <= l < mhi have no lambda: zero them
+  {
+    const int nz = ((st.mhi < lmax + 1 ? st.mhi : lmax + 1) - m) * ncol;
+    for (int e = threadIdx.x; e < nz; e += blockDim.x)
+      prow[(int64_t)(m + e / ncol) * ncol + e % ncol] = 0.f;
+  }
+  if (st.mhi > lmax) return;
+  const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
+
+  // B: P[0] = N + sg*S and P[1] = N - sg*S (columns Q.re, Q.im, U.re, U.im)
+  float bp[2][16][NCB];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int rs = wbase + 4 * j + kq;
+    const bool ok = rs < nslots && a.mlim[rs < nslots ? rs : 0] >= m;
+    const bool pr = ok && a.ispair[rs] != 0;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n;
+      double nv = 0., sv = 0.;
+      if (ok && col < ncol) {
+        const int comp = 2 * a.t0 + (col >> 1), part = col & 1;
+        nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
+        if (pr) sv = sg * ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
+      }
+      bp[0][j][cb] = (float)(nv + sv);
+      bp[1][j][cb] = (float)(nv - sv);
+    }
+  }
+  // column 4t+q of the lambda_x product takes P's column 4t+3-q with this sign
+  // (a(E,B) += lw*P[x] + lx*rot(P[x^1]), libsharp/sharp_core_inc2.c:339-364)
+  const float rsgn = ((n & 3) == 0 || (n & 3) == 3) ? -1.f : 1.f;
+
+  const double cth = a.cth[s], sth = a.sth[s];
+  const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+  const int lpad = lmax + kTabPad;
+  auto fx = [&](int l) {
+    const int k = min(l, lpad);
+    return Fx{fxb[3 * k], fxb[3 * k + 1], fxb[3 * k + 2]};
+  };
+  double rec1p = 0., rec1m = 0., rec2p, rec2m;
+  int scalep, scalem;
+  spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep, scalem);
+  int mode = !wave_any(active) ? 3 : (wave_all(scalep < 1 && scalem < 1) ? 0 : 1);
+  double cfp = corfac(scalep), cfm = corfac(scalem);
+  int l = st.mhi;
+  float *mw = lds[w][0], *mx = lds[w][1];
+  float *slices[W];
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
+  // coefficients fx(l+1..l+4) of the current two two-steps, prefetched as in
+  // the synthesis kernels
+  Fx fc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
+
+  const int nchunks = (lmax - st.mhi + 32) / 32;
+  for (int c = 0; c < nchunks; ++c) {
+    const int lc = st.mhi + 32 * c;
+    bool any = false;
+#pragma unroll
+    for (int g4 = 0; g4 < 8; ++g4) {
+      const int lb = l;
+      Fx fn[4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = 2 * g4 + h;
+        double lw0 = 0., lx0 = 0., lw1 = 0., lx1 = 0.;
+        if (mode != 3 && l <= lmax) {
+          if (mode == 0) {
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+            rescale1(rec1p, rec2p, scalep);
+            rescale1(rec1m, rec2m, scalem);
+            if (wave_any(scalep >= 1 || scalem >= 1)) {
+              mode = 1;
+              cfp = corfac(scalep); cfm = corfac(scalem);
+            }
+          } else {
+            any = true;
+            const double rp0 = rec2p * cfp, rm0 = rec2m * cfm;
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+            const double rp1 = rec1p * cfp, rm1 = rec1m * cfm;
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+            lw0 = rp0 + rm0;
+            lx0 = rm0 - rp0;
+            if (l + 1 <= lmax) {
+              lw1 = rp1 + rm1;
+              lx1 = rm1 - rp1;
+            }
+            if (mode == 1) {
+              const bool a1 = rescale1(rec1p, rec2p, scalep);
+              const bool a2 = rescale1(rec1m, rec2m, scalem);
+              if (a1 || a2) { cfp = corfac(scalep); cfm = corfac(scalem); }
+              if (wave_all(scalep >= 0 && scalem >= 0)) {
+                rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+                cfp = cfm = 1.;
+                mode = 2;
+              }
+            }
+          }
+        }
+        mw[lane * kLS + 2 * j] = active ? (float)lw0 : 0.f;
+        mw[lane * kLS + 2 * j + 1] = active ? (float)lw1 : 0.f;
+        mx[lane * kLS + 2 * j] = active ? (float)lx0 : 0.f;
+        mx[lane * kLS + 2 * j + 1] = active ? (float)lx1 : 0.f;
+        l += 2;
+        if (h == 0) {
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) fn[k] = fx(lb + 5 + k);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fc[k] = fn[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    f4 tw[2][NCB], tx[2][NCB];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) tw[x][cb] = tx[x][cb] = f4{0.f, 0.f, 0.f, 0.f};
+    if (any) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int off = (4 * j + kq) * kLS + x + 2 * n;
+          const float Aw = mw[off], Ax = mx[off];
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            tw[x][cb] = mfma4(Aw, bp[x][j][cb], tw[x][cb]);
+            tx[x][cb] = mfma4(Ax, bp[x ^ 1][j][cb], tx[x][cb]);
+          }
+        }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            tw[x][cb][v] += rsgn * __shfl_xor(tx[x][cb][v], 3);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    write_tiles<NCB>(mw, tw);
+    __syncthreads();
+    reduce_store<2 * NCB>(slices, W, prow, lc, lmax, ncol);
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static void check_launch(const char *what)
+{
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "kernel launch failed (%s): %s", what,
+             hipGetErrorString(e));
+    SHP_FAIL(buf);
+  }
+}
+
+void launch_msynth(const LegArgs &a, hipStream_t st)
+{
+  const int nsb = (a.nslots + 63) / 64;
+  const dim3 grid(xcd_grid(a.nmb, nsb));
+  const dim3 block(64, MB);
+  const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
+  if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
+  const bool two = cols > 16;
+  if (a.spin == 0) {
+    if (two) hipLaunchKernelGGL(k_msynth_spin0<2>, grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
+  } else if (a.deriv1) {
+    if (two) hipLaunchKernelGGL((k_msynth_spin<2, true>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_msynth_spin<1, true>), grid, block, 0, st, a, nsb);
+  } else {
+    if (two) hipLaunchKernelGGL((k_msynth_spin<2, false>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_msynth_spin<1, false>), grid, block, 0, st, a, nsb);
+  }
+  check_launch("mixed synthesis");
+}
+
+void launch_manal(const LegArgs &a, hipStream_t st)
+{
+  const dim3 grid(xcd_grid(a.mmax + 1, a.ngroups));
+  const dim3 block(64 * kMixWaves);
+  const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
+  if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
+  const bool two = cols > 16;
+  if (a.spin == 0) {
+    if (two) hipLaunchKernelGGL(k_manal_spin0<2>, grid, block, 0, st, a);
+    else hipLaunchKernelGGL(k_manal_spin0<1>, grid, block, 0, st, a);
+  } else {
+    if (two) hipLaunchKernelGGL(k_manal_spin<2>, grid, block, 0, st, a);
+    else hipLaunchKernelGGL(k_manal_spin<1>, grid, block, 0, st, a);
+  }
+  check_launch("mixed analysis");
+}
+
+} // namespace shp

@@ -246,14 +246,18 @@ void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStr
 }
 
 // ---------------------------------------------------------------------------
-// a_lm gather / scatter.  grid: (l tiles, mmax+1, ncomp)
+// a_lm gather / scatter.  grid: (tiles of (l, component) pairs of one m, m);
+// element e of row m is l = m + e / ncomp, component e % ncomp, so the
+// canonical rows are read and written contiguously.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
 {
-  const int m = blockIdx.y, comp = blockIdx.z;
-  const int l = m + blockIdx.x * 256 + threadIdx.x;
-  if (l > a.lmax) return;
+  const int m = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int nc = a.ncomp;
+  if (e >= (int64_t)(a.lmax - m + 1) * nc) return;
+  const int l = m + (int)(e / nc), comp = (int)(e % nc);
   const bool realonly = a.packed && m == 0;
   int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
   if (!a.packed) ofs *= 2;
@@ -263,22 +267,42 @@ __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
   double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
   double f = a.norm_l[l];
   if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
-  a.almtmp[(tri_ofs(m, a.lmax) + l - m) * a.ncomp + comp] = make_double2(re * f, im * f);
+  a.almtmp[(int64_t)tri_ofs(m, a.lmax) * nc + e] = make_double2(re * f, im * f);
 }
 
+// components [c0, c0+npc); the values come from almtmp, or (part != nullptr)
+// from the partial sums of the ring-pair groups that reach m, added in FP64
+// in group order
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
 {
-  const int m = blockIdx.y, comp = blockIdx.z;
-  const int l = m + blockIdx.x * 256 + threadIdx.x;
-  if (l > a.lmax) return;
+  const int m = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int npc = a.npc;
+  if (e >= (int64_t)(a.lmax - m + 1) * npc) return;
+  const int l = m + (int)(e / npc), cl = (int)(e % npc), comp = a.c0 + cl;
+  const int64_t row = tri_ofs(m, a.lmax) + l - m;
+  double2 x;
+  if (a.part) {
+    const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * 2 * npc;
+    const float *pp = a.part + row * 2 * npc + 2 * cl;
+    double re = 0., im = 0.;
+    for (int g = 0; g < a.ngroups; ++g) {
+      if (a.gmax[g] < m) continue;
+      const float2 v = *reinterpret_cast<const float2 *>(pp + g * gstride);
+      re += v.x;
+      im += v.y;
+    }
+    x = make_double2(re, im);
+  } else {
+    x = a.almtmp[row * a.ncomp + comp];
+  }
   const bool realonly = a.packed && m == 0;
   int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
   if (!a.packed) ofs *= 2;
   if (!realonly) stride *= 2;
   const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
   void *p = a.alms.p[comp];
-  const double2 x = a.almtmp[(tri_ofs(m, a.lmax) + l - m) * a.ncomp + comp];
   double f = a.norm_l[l];
   if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrtHalf;
   const bool add = (a.flags & SHARP_ADD) != 0;
@@ -293,7 +317,8 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
 
 void launch_alm_gather(const AlmArgs &a, hipStream_t s)
 {
-  const dim3 grid((a.lmax + 256) / 256, a.mmax + 1, a.ncomp);
+  const int64_t per = (int64_t)(a.lmax + 1) * a.ncomp;
+  const dim3 grid((unsigned)((per + 255) / 256), a.mmax + 1);
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_gather<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_gather<float>, grid, dim3(256), 0, s, a);
   check_launch("alm_gather");
@@ -301,7 +326,8 @@ void launch_alm_gather(const AlmArgs &a, hipStream_t s)
 
 void launch_alm_scatter(const AlmArgs &a, hipStream_t s)
 {
-  const dim3 grid((a.lmax + 256) / 256, a.mmax + 1, a.ncomp);
+  const int64_t per = (int64_t)(a.lmax + 1) * a.npc;
+  const dim3 grid((unsigned)((per + 255) / 256), a.mmax + 1);
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_scatter<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_scatter<float>, grid, dim3(256), 0, s, a);
   check_launch("alm_scatter");

@@ -32,6 +32,7 @@ namespace shp {
 
 constexpr int MB = 8;        // m values per phase block
 constexpr int kTabPad = 8;   // zero entries appended to every table row
+constexpr int kAlmSlackRows = 16;   // rows after the last canonical a_lm row
 
 struct RingDev {
   int64_t ofs;     // index of the first pixel in the map
