@@ -6,8 +6,8 @@
  *
  * Conventions: `alm` and `map` are host arrays of DEVICE pointers with the
  * same meaning and count as in sharp_execute (libsharp/sharp_lowlevel.h:
- * 196-224); `stream` is a hipStream_t (NULL: the library's own stream on the
- * current device).  The call enqueues the work and returns; the caller
+ * 196-224); `stream` is a hipStream_t (NULL: the null stream of the current
+ * device, as in HIP).  The call enqueues the work and returns; the caller
  * synchronises the stream before reading the outputs.  Argument errors
  * behave as in sharp_execute (message, then exit(1)).
  */
@@ -50,7 +50,7 @@ void sharp_hip_map2alm_adjoint_device (int spin, void *alm, void *map,
 void sharp_hip_prepare (const sharp_geom_info *geom_info,
   const sharp_alm_info *alm_info, int spin);
 
-/* Waits for all work the library enqueued on `stream` (NULL: its own). */
+/* Waits for the work enqueued on `stream` (NULL: the null stream). */
 void sharp_hip_synchronize (void *stream);
 
 /* Releases every cached table and work buffer of the current device. */

@@ -58,7 +58,11 @@ struct DevBuf {
   {
     if (bytes == 0) bytes = 16;
     if (bytes > cap) {
-      if (p) HIP_CHECK(hipFree(p));
+      // the old buffer may still be in use by work queued on any stream
+      if (p) {
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipFree(p));
+      }
       p = nullptr;
       HIP_CHECK(hipMalloc(&p, bytes));
       cap = bytes;
@@ -97,7 +101,7 @@ struct GeomPlan {
     int lmax, spin;
     std::vector<int> h;
     int *d = nullptr;
-    // largest mlim of each group of kMixGroupSlots slots (mixed analysis)
+    // largest mlim of each group of mix_group_slots(spin) slots (mixed analysis)
     std::vector<int> hgmax;
     int *gmax = nullptr;
   };
@@ -105,7 +109,9 @@ struct GeomPlan {
   // ring transforms: FFT plans grouped by LDS class, the rings too long for
   // any class (direct summation), and the list of all rings (SHARP_NO_FFT)
   RingFftPlan *fft[2 * kNumFftClasses] = {};   // [2*class + bluestein]
+  std::vector<RingFftPlan> hfft[2 * kNumFftClasses];
   int nfft[2 * kNumFftClasses] = {};
+  double2 *btab = nullptr;   // Bluestein chirp spectra
   int *dft_list = nullptr, ndft = 0;
   int *all_list = nullptr, nall = 0;
   int64_t npix_fft = 0, npix_all = 0;   // pixels in FFT rings / in all rings
@@ -114,6 +120,7 @@ struct GeomPlan {
     (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
     for (auto &m : mlims) { (void)hipFree(m.d); (void)hipFree(m.gmax); }
     for (auto *p : fft) (void)hipFree(p);
+    (void)hipFree(btab);
     (void)hipFree(dft_list);
     (void)hipFree(all_list);
   }
@@ -262,11 +269,11 @@ static int fft_plan(int rid, int n, RingFftPlan &pl)
     pl.bluestein = 0;
     need = (size_t)pl.N;
   } else {
+    // smallest 7-smooth convolution length >= 2N-1
     pl.bluestein = 1;
-    size_t M = 1;
-    while (M < 2 * (size_t)pl.N - 1) M <<= 1;
-    pl.M = (int)std::min<size_t>(M, 1u << 30);
-    radices(pl.M, rad);
+    size_t M = 2 * (size_t)pl.N - 1;
+    while (M < (1u << 30) && radices((int)M, rad) != 1) ++M;
+    pl.M = (int)M;
     need = M;
   }
   if (rad.size() > sizeof(pl.rad)) return -1;
@@ -344,7 +351,8 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     all.push_back(rid);
     RingFftPlan pl;
     int cl = fft_plan(rid, n, pl);
-    if (cl >= 0 && (size_t)(512 << cl) * sizeof(double2) + 64 > (size_t)c.max_lds)
+    // the class's FFT buffer plus room to stage 512 phases (ring_fft.hip)
+    if (cl >= 0 && (size_t)((512 << cl) + 512) * sizeof(double2) + 256 > (size_t)c.max_lds)
       cl = -1;
     gp->npix_all += n;
     if (cl < 0) dft.push_back(rid);
@@ -353,9 +361,46 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
       gp->npix_fft += n;
     }
   }
+  // Bluestein chirp spectra: one table entry per distinct (N, M)
+  std::map<std::pair<int, int>, int64_t> boffs;
+  int64_t btot = 0;
+  FftClasses uniq{};
+  std::vector<RingFftPlan> ucls[2 * kNumFftClasses];
   for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
+    for (auto &pl : cls[c2]) {
+      pl.boff = -1;
+      if (!pl.bluestein) continue;
+      auto key = std::make_pair(pl.N, pl.M);
+      auto it = boffs.find(key);
+      if (it == boffs.end()) {
+        it = boffs.emplace(key, btot).first;
+        btot += pl.M;
+        RingFftPlan u = pl;
+        u.boff = it->second;
+        ucls[c2].push_back(u);
+      }
+      pl.boff = it->second;
+    }
+    // longest rings first: they take longest, and the rings that need the
+    // phase staging of the synthesis fold (n <= mmax) come last
+    std::stable_sort(cls[c2].begin(), cls[c2].end(),
+                     [](const RingFftPlan &x, const RingFftPlan &y) { return x.n > y.n; });
     gp->nfft[c2] = (int)cls[c2].size();
     if (!cls[c2].empty()) gp->fft[c2] = upload(cls[c2]);
+    gp->hfft[c2] = cls[c2];
+  }
+  if (btot > 0) {
+    HIP_CHECK(hipMalloc(&gp->btab, (size_t)btot * sizeof(double2)));
+    std::vector<RingFftPlan *> dev(2 * kNumFftClasses, nullptr);
+    for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
+      uniq.count[c2] = (int)ucls[c2].size();
+      if (!ucls[c2].empty()) dev[c2] = upload(ucls[c2]);
+      uniq.plans[c2] = dev[c2];
+      uniq.hplans[c2] = ucls[c2].data();
+    }
+    launch_bluestein_table(uniq, gp->btab, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (auto *d : dev) (void)hipFree(d);
   }
   gp->ndft = (int)dft.size();
   gp->dft_list = upload(dft);
@@ -380,10 +425,10 @@ static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
   for (int k = 0; k < gp.nslots; ++k)
     ml.h[k] = sharp_get_mlim(lmax, spin, gp.hsth[k], gp.hcth[k]);
   ml.d = upload(ml.h);
-  const int ng = (gp.nslots + kMixGroupSlots - 1) / kMixGroupSlots;
+  const int gs = mix_group_slots(spin);
+  const int ng = (gp.nslots + gs - 1) / gs;
   ml.hgmax.assign(ng, -1);
-  for (int k = 0; k < gp.nslots; ++k)
-    ml.hgmax[k / kMixGroupSlots] = std::max(ml.hgmax[k / kMixGroupSlots], ml.h[k]);
+  for (int k = 0; k < gp.nslots; ++k) ml.hgmax[k / gs] = std::max(ml.hgmax[k / gs], ml.h[k]);
   ml.gmax = upload(ml.hgmax);
   gp.mlims.push_back(std::move(ml));
   return gp.mlims.back();
@@ -665,6 +710,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   ra.phase = phase;
   ra.nphmax = gp.nphmax;
   ra.dp = (j.flags & SHARP_DP) ? 1 : 0;
+  ra.max_lds = c.max_lds;
   for (int i = 0; i < ncomp_map; ++i) ra.maps.p[i] = dmap[i];
 
   LegArgs la{};
@@ -710,8 +756,10 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
 
   FftClasses fc{};
   int64_t nring_fft = 0, npix_fft = 0, npix_all = 0;
+  fc.btab = gp.btab;
   for (int cl = 0; cl < 2 * kNumFftClasses; ++cl) {
     fc.plans[cl] = gp.fft[cl];
+    fc.hplans[cl] = gp.hfft[cl].data();
     fc.count[cl] = gp.nfft[cl];
     nring_fft += gp.nfft[cl];
   }
@@ -899,7 +947,9 @@ void sharp_hip_execute_device(sharp_jobtype type, int spin, void *alm, void *map
   std::lock_guard<std::mutex> lock(g_mutex);
   const Job j = make_job(type, spin, geom_info, alm_info, ntrans, flags);
   Ctx &c = context();
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  // NULL is the device's null (default) stream, as everywhere in HIP; the
+  // library's own stream is only used by the host-pointer sharp_execute
+  hipStream_t st = static_cast<hipStream_t>(stream);
   run_device(c, j, static_cast<void *const *>(alm), static_cast<void *const *>(map),
              0, 0, st, opcnt);
 }
@@ -939,7 +989,8 @@ void sharp_hip_synchronize(void *stream)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
   Ctx &c = context();
-  HIP_CHECK(hipStreamSynchronize(stream ? static_cast<hipStream_t>(stream) : c.stream));
+  (void)c;
+  HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
 }
 
 void sharp_hip_release(void)

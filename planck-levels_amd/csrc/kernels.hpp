@@ -39,8 +39,11 @@ void launch_anal(const LegArgs &a, int ntg, hipStream_t st);
 // of up to kMixMaxT transforms per launch (a.ntl).
 constexpr int kMixMaxT0 = 16;   // spin 0: 2 real columns per transform
 constexpr int kMixMaxTS = 8;    // spin s: 4 real columns per transform
-constexpr int kMixWaves = 8;    // analysis: wavefronts per workgroup
-constexpr int kMixGroupSlots = 64 * kMixWaves;
+// analysis: wavefronts per workgroup (spin 0 / spin s); a workgroup sums the
+// ring pairs of one group of 64*waves slots
+constexpr int kMixWaves0 = 8;
+constexpr int kMixWavesS = 4;
+constexpr int mix_group_slots(int spin) { return 64 * (spin == 0 ? kMixWaves0 : kMixWavesS); }
 void launch_msynth(const LegArgs &a, hipStream_t st);
 void launch_manal(const LegArgs &a, hipStream_t st);
 
@@ -60,6 +63,7 @@ struct RingArgs {
   double2 *phase;
   int nphmax;
   int dp;               // 1: double maps, 0: float maps
+  int max_lds;          // bytes of LDS a workgroup may allocate
   PtrTab maps;          // [ncomp] device pointers (float or double)
 };
 // Direct-summation ring transforms over the rings in `list` (nlist entries of
@@ -75,16 +79,22 @@ struct RingFftPlan {
   int n;           // pixels
   int N;           // complex FFT length: n/2 (even n) or n (odd n)
   int bluestein;   // 1: N has a prime factor > 7
-  int M;           // Bluestein convolution length (power of two >= 2N-1)
+  int M;           // Bluestein convolution length (7-smooth, >= 2N-1)
   int nrad;
   int8_t rad[16];
+  int64_t boff;    // Bluestein: offset of the chirp spectrum in the table
 };
 constexpr int kNumFftClasses = 5;   // LDS buffers of 512 << c complex doubles
-// plans[2*class + bluestein]
+// plans[2*class + bluestein], sorted by decreasing ring length; hplans are
+// host copies of the same arrays
 struct FftClasses {
   const RingFftPlan *plans[2 * kNumFftClasses];
+  const RingFftPlan *hplans[2 * kNumFftClasses];
   int count[2 * kNumFftClasses];
+  const double2 *btab;   // Bluestein chirp spectra
 };
+// Fills btab at the offsets of the given plans (one plan per distinct (N, M)).
+void launch_bluestein_table(const FftClasses &uniq, double2 *btab, hipStream_t st);
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st);
 
 // a_lm in the caller's layout <-> canonical almtmp.

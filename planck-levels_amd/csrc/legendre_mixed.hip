@@ -20,7 +20,7 @@
 // 282-306).  The MFMA accumulators are emptied into FP32 totals every 8 K
 // slices, which keeps the rounding error of long sums near unit roundoff.
 //
-// Analysis: a workgroup of kMixWaves wavefronts per (m, group of 64*kMixWaves
+// Analysis: a workgroup of kMixWaves0/S wavefronts per (m, group of 64*waves
 // ring pairs).  Each lane runs the recurrence for its ring pair 32 l at a time
 // and stores lambda (FP32) in its wavefront's LDS slice; the MFMA reads it back
 // transposed, A = [16 l of one parity x 4 ring pairs], against
@@ -670,9 +670,9 @@ __device__ inline void write_tiles(float *slice, const f4 (&t)[2][NCB])
 }
 
 template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWaves) k_manal_spin0(LegArgs a)
+__global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
 {
-  constexpr int W = kMixWaves;
+  constexpr int W = kMixWaves0;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   __shared__ float lds[W][64 * kLS];
   int m, g;
@@ -817,9 +817,9 @@ __global__ void __launch_bounds__(64 * kMixWaves) k_manal_spin0(LegArgs a)
 }
 
 template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWaves) k_manal_spin(LegArgs a)
+__global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
 {
-  constexpr int W = kMixWaves;
+  constexpr int W = kMixWavesS;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   __shared__ float lds[W][2][64 * kLS];
   int m, g;
@@ -1033,7 +1033,7 @@ void launch_msynth(const LegArgs &a, hipStream_t st)
 void launch_manal(const LegArgs &a, hipStream_t st)
 {
   const dim3 grid(xcd_grid(a.mmax + 1, a.ngroups));
-  const dim3 block(64 * kMixWaves);
+  const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
   if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
   const bool two = cols > 16;

@@ -31,6 +31,8 @@
 #include "sharp_lowlevel.h"
 #include "devmath.hpp"
 
+#include <algorithm>
+
 namespace shp {
 
 namespace {
@@ -242,10 +244,9 @@ __device__ inline double2 chirp(int64_t j, int N, int sign)
 }
 
 // FFT of the convolution kernel b_d = e^{-sign i pi d^2 / N}, |d| < N,
-// wrapped to length M; returns this thread's 8 values (index tid + q*NT).
+// wrapped to length M, left in buf[0..M)
 template <int NT, int E>
-__device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign,
-                                 double2 (&B)[E])
+__device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
 {
   const int N = P.N, M = P.M;
   for (int j = threadIdx.x; j < M; j += NT) {
@@ -256,18 +257,13 @@ __device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign,
   }
   __syncthreads();
   lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    B[q] = (idx < M) ? buf[idx] : make_double2(0., 0.);
-  }
-  __syncthreads();
 }
 
-// length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein
+// length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein;
+// Bt is the spectrum of the Bluestein kernel (see bluestein_kernel), read
+// from the table in global memory
 template <int NT, int E, bool BLUE>
-__device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign,
-                         const double2 (&B)[BLUE ? E : 1])
+__device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
   if constexpr (!BLUE) {
     lds_fft<NT, E>(buf, P.N, P.rad, P.nrad, sign);
@@ -277,10 +273,13 @@ __device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign,
     buf[j] = (j < N) ? cmul(buf[j], chirp(j, N, sign)) : make_double2(0., 0.);
   __syncthreads();
   lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    if (idx < M) buf[idx] = cmul(buf[idx], B[q]);
+  // the kernel of sign s is the conjugate of the kernel of sign -s with the
+  // input conjugated: FFT(conj(x))_k = conj(FFT(x)_{-k}); the table holds
+  // the spectrum for the analysis sign -1
+  for (int j = threadIdx.x; j < M; j += NT) {
+    double2 b = Bt[j];
+    if (sign > 0) b = conj(Bt[j == 0 ? 0 : M - j]);
+    buf[j] = cmul(buf[j], b);
   }
   __syncthreads();
   lds_fft<NT, E>(buf, M, P.rad, P.nrad, +1);
@@ -301,20 +300,45 @@ __device__ inline double2 group_sum(double2 v, int G)
   return v;
 }
 
-// threads per ring: S/8 (8 values per thread and pass), at most 512
-template <int S> constexpr int kThreads = (S / 8 < 512) ? S / 8 : 512;
+// threads per ring: S/8 (8 values per thread and pass) within [256, 512];
+// small rings still read all mmax+1 phases, which wants many threads
+template <int S> constexpr int kThreads = (S / 8 < 256) ? 256 : ((S / 8 < 512) ? S / 8 : 512);
+#define SHP_FFT_OCC
+
+// Runs consume(i, load(i)) for i = lo+tid, lo+tid+NT, ... < hi with U loads
+// in flight per thread (all U are issued before the first is used).
+template <int NT, int U, typename L, typename C>
+__device__ inline void batched(int lo, int hi, L &&load, C &&consume)
+{
+  for (int base = lo + (int)threadIdx.x; base < hi; base += NT * U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * NT;
+      v[u] = (i < hi) ? load(i) : make_double2(0., 0.);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * NT;
+      if (i < hi) consume(i, v[u]);
+    }
+  }
+}
 
 } // namespace
 
 // ===========================================================================
 // synthesis: phase -> ring
 // ===========================================================================
-template <int S, bool BLUE, typename T>
-__global__ void __launch_bounds__(kThreads<S>) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
-                                                          int comps_per_block)
+template <int S, bool BLUE, bool STAGED, typename T>
+__global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
+                                                          int comps_per_block, int vcap,
+                                                          const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
-  __shared__ double2 buf[S];
+  // buf: the ring's FFT buffer; V (STAGED): up to vcap weighted, rotated phases
+  extern __shared__ double2 smem[];
+  double2 *buf = smem, *V = smem + S;
   __shared__ double2 nyq;
   const RingFftPlan &P = plans[blockIdx.x];
   const int tid = threadIdx.x;
@@ -325,55 +349,95 @@ __global__ void __launch_bounds__(kThreads<S>) k_phase2ring_fft(RingArgs a, cons
   double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
   if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
   const bool norot = fabs(r.phi0) < 1e-14;
-
-  double2 B[BLUE ? E : 1];
-  if constexpr (BLUE) bluestein_kernel<NT, E>(buf, P, +1, B);
+  const double2 *Bt = BLUE ? btab + P.boff : nullptr;
+  // w e^{i m phi0} phase_m, real part only for m = 0
+  auto cval = [&](int m, double2 v) {
+    v = cscale(v, wgt);
+    if (m == 0) v.y = 0.;
+    else if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
+    return v;
+  };
 
   // bins: even n -> C_0..C_N, odd n -> C_0..C_{(n-1)/2}
   const int nbins = even ? N + 1 : (n + 1) / 2;
   int G = 1;
   while (G < 64 && 2 * G * nbins <= NT) G <<= 1;
   const int bins_per_round = NT / G;
+  auto put_bin = [&](int k, double2 s, bool first) {
+    if (even) {
+      double2 &dst = (k < N) ? buf[k] : nyq;
+      dst = first ? s : cadd(dst, s);
+    } else {
+      buf[k] = first ? s : cadd(buf[k], s);
+      if (k > 0) buf[n - k] = conj(buf[k]);
+    }
+  };
 
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
-    // 1. fold
-    for (int kb = 0; kb < nbins; kb += bins_per_round) {
-      const int k = kb + tid / G, part = tid % G;
-      double2 s = make_double2(0., 0.);
-      if (k < nbins) {
-        for (int64_t m = k + (int64_t)part * n; m <= mmax; m += (int64_t)G * n) {
-          double2 v = a.phase[phase_index(comp, (int)m, slot, hemi, a.nslots, a.nmb)];
-          v = cscale(v, wgt);
-          if (m == 0) {
-            s.x += v.x;
-          } else {
-            if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
-            s = cadd(s, v);
-          }
+    // 1. fold: C_k = sum over m = k (mod n) of c_m + conj(sum over m = -k
+    //    (mod n), m > 0, of c_m)
+    if constexpr (!STAGED) {
+      // n > mmax: at most c_k and conj(c_{n-k}) per bin; 4 bins (8 loads)
+      // in flight per thread
+      for (int base = tid; base < nbins; base += 4 * NT) {
+        double2 vp[4], vn[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = base + u * NT, mn = (k == 0) ? n : n - k;
+          vp[u] = (k < nbins && k <= mmax)
+                      ? a.phase[phase_index(comp, k, slot, hemi, a.nslots, a.nmb)]
+                      : make_double2(0., 0.);
+          vn[u] = (k < nbins && mn <= mmax)
+                      ? a.phase[phase_index(comp, mn, slot, hemi, a.nslots, a.nmb)]
+                      : make_double2(0., 0.);
         }
-        for (int64_t m = (int64_t)(1 + part) * n - k; m <= mmax; m += (int64_t)G * n) {
-          if (m < 1) continue;
-          double2 v = a.phase[phase_index(comp, (int)m, slot, hemi, a.nslots, a.nmb)];
-          v = cscale(v, wgt);
-          if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
-          s = cadd(s, conj(v));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = base + u * NT, mn = (k == 0) ? n : n - k;
+          if (k >= nbins) continue;
+          double2 sum = make_double2(0., 0.);
+          if (k <= mmax) sum = cval(k, vp[u]);
+          if (mn <= mmax) sum = cadd(sum, conj(cval(mn, vn[u])));
+          put_bin(k, sum, true);
         }
       }
-      s = group_sum(s, G);
-      if (part == 0 && k < nbins) {
-        if (even) {
-          if (k < N) buf[k] = s;
-          else nyq = s;
-        } else {
-          buf[k] = s;
-          if (k > 0) buf[n - k] = conj(s);
+      __syncthreads();
+    } else {
+      // stage vcap phases at a time in LDS, then sum each bin's aliases
+      // with G lanes in a fixed order
+      for (int m0 = 0; m0 <= mmax; m0 += vcap) {
+        const int m1 = min(mmax + 1, m0 + vcap);
+        batched<NT, 8>(m0, m1,
+            [&](int m) { return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)]; },
+            [&](int m, double2 v) { V[m - m0] = cval(m, v); });
+        __syncthreads();
+        for (int kb = 0; kb < nbins; kb += bins_per_round) {
+          const int k = kb + tid / G, part = tid % G;
+          double2 sum = make_double2(0., 0.);
+          if (k < nbins) {
+            // first m >= m0 with m = k (mod n), then every G-th alias
+            int64_t mp = k;
+            if (mp < m0) mp += ((m0 - k + n - 1) / n) * (int64_t)n;
+            for (int64_t m = mp + (int64_t)part * n; m < m1; m += (int64_t)G * n)
+              sum = cadd(sum, V[m - m0]);
+            // m = -k (mod n), m >= max(m0, 1)
+            int64_t mn = (k == 0) ? n : n - k;
+            const int lo = max(m0, 1);
+            if (mn < lo) mn += ((lo - mn + n - 1) / n) * (int64_t)n;
+            double2 t = make_double2(0., 0.);
+            for (int64_t m = mn + (int64_t)part * n; m < m1; m += (int64_t)G * n)
+              t = cadd(t, V[m - m0]);
+            sum = cadd(sum, conj(t));
+          }
+          sum = group_sum(sum, G);
+          if (part == 0 && k < nbins) put_bin(k, sum, m0 == 0);
         }
+        __syncthreads();
       }
     }
-    __syncthreads();
-    // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
+  // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
     if (even) {
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
@@ -390,7 +454,7 @@ __global__ void __launch_bounds__(kThreads<S>) k_phase2ring_fft(RingArgs a, cons
       __syncthreads();
     }
     // 3. inverse FFT of length N
-    ring_dft<NT, E, BLUE>(buf, P, +1, B);
+    ring_dft<NT, E, BLUE>(buf, P, +1, Bt);
     // 4. store
     void *mp = a.maps.p[comp];
     const bool add = (a.flags & SHARP_ADD) != 0;
@@ -414,11 +478,13 @@ __global__ void __launch_bounds__(kThreads<S>) k_phase2ring_fft(RingArgs a, cons
 // analysis: ring -> phase
 // ===========================================================================
 template <int S, bool BLUE, typename T>
-__global__ void __launch_bounds__(kThreads<S>) k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
-                                                          int comps_per_block)
+__global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
+                                                          int comps_per_block, int,
+                                                          const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
-  __shared__ double2 buf[S];
+  extern __shared__ double2 smem[];
+  double2 *buf = smem;
   __shared__ double2 nyq;
   const RingFftPlan &P = plans[blockIdx.x];
   const int tid = threadIdx.x;
@@ -430,26 +496,26 @@ __global__ void __launch_bounds__(kThreads<S>) k_ring2phase_fft(RingArgs a, cons
   if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
   const bool norot = fabs(r.phi0) < 1e-14;
 
-  double2 B[BLUE ? E : 1];
-  if constexpr (BLUE) bluestein_kernel<NT, E>(buf, P, -1, B);
+  const double2 *Bt = BLUE ? btab + P.boff : nullptr;
 
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
     const void *mp = a.maps.p[comp];
     // 1. load: even n -> z_j = f_2j + i f_2j+1, odd n -> z_j = f_j
-    for (int j = tid; j < N; j += NT) {
-      if (even) {
-        const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
-        buf[j] = make_double2(ld<T>(mp, i0), ld<T>(mp, i0 + r.stride));
-      } else {
-        const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
-        buf[j] = make_double2(ld<T>(mp, i0), 0.);
-      }
-    }
+    batched<NT, 8>(0, N,
+        [&](int j) {
+          if (even) {
+            const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
+            return make_double2(ld<T>(mp, i0), ld<T>(mp, i0 + r.stride));
+          }
+          const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
+          return make_double2(ld<T>(mp, i0), 0.);
+        },
+        [&](int j, double2 v) { buf[j] = v; });
     __syncthreads();
     // 2. forward FFT
-    ring_dft<NT, E, BLUE>(buf, P, -1, B);
+    ring_dft<NT, E, BLUE>(buf, P, -1, Bt);
     // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
     if (even) {
       for (int k = tid; 2 * k <= N; k += NT) {
@@ -489,6 +555,12 @@ __global__ void __launch_bounds__(kThreads<S>) k_ring2phase_fft(RingArgs a, cons
 // ===========================================================================
 // launchers: one launch per LDS size class
 // ===========================================================================
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    const hipError_t e_ = (x);                                             \
+    if (e_ != hipSuccess) SHP_FAIL(hipGetErrorString(e_));                 \
+  } while (0)
+
 static void check_launch(const char *what)
 {
   const hipError_t e = hipGetLastError();
@@ -500,33 +572,91 @@ static void check_launch(const char *what)
   }
 }
 
+// Bluestein kernel spectra for the analysis sign, one plan per workgroup
 template <int S>
-static void launch_class(bool synth, bool blue, const RingArgs &a,
-                         const RingFftPlan *plans, int nplans, hipStream_t st)
+__global__ void __launch_bounds__(kThreads<S>) k_bluestein_table(const RingFftPlan *plans,
+                                                           double2 *btab)
 {
+  constexpr int NT = kThreads<S>, E = S / NT;
+  extern __shared__ double2 smem[];
+  const RingFftPlan &P = plans[blockIdx.x];
+  bluestein_kernel<NT, E>(smem, P, -1);
+  for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = smem[j];
+}
+
+template <typename K> static void set_lds(K kern, size_t lds)
+{
+  if (lds > 65536)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+static size_t plan_need(const RingFftPlan &p) { return p.bluestein ? p.M : p.N; }
+
+template <int S>
+static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
+                         const RingFftPlan *plans, const RingFftPlan *hplans, int nplans,
+                         const double2 *btab, hipStream_t st)
+{
+  if (nplans <= 0) return;
   // split components over blocks only when there are few rings
   int cpb = a.ncomp;
   while (cpb > 1 && (int64_t)nplans * ((a.ncomp + cpb - 1) / cpb) < 2048) cpb = (cpb + 1) / 2;
   const dim3 grid(nplans, (a.ncomp + cpb - 1) / cpb);
   const dim3 block(kThreads<S>);
+  size_t need = 0;
+  for (int i = 0; i < nplans; ++i) need = std::max(need, plan_need(hplans[i]));
+  // the staging area V starts after the class's S-entry buffer
+  const size_t fbytes = (staged ? (size_t)S : need) * sizeof(double2);
+  // staged synthesis keeps up to vcap phases next to the FFT buffer: the
+  // whole ring's mmax+1 when the LDS allows, else it loops
+  int vcap = 0;
+  if (staged) {
+    const size_t budget = (S >= 4096) ? (size_t)a.max_lds - 256
+                                      : std::min<size_t>(81920, (size_t)a.max_lds - 256);
+    vcap = (int)std::min<size_t>((size_t)a.mmax + 1, (budget - fbytes) / sizeof(double2));
+    if (vcap < 1) SHP_FAIL("ring FFT: no LDS left for the phase staging");
+  }
+  const size_t lds = fbytes + (size_t)vcap * sizeof(double2);
+  auto go = [&](auto kern) {
+    set_lds(kern, lds);
+    hipLaunchKernelGGL(kern, grid, block, lds, st, a, plans, cpb, vcap, btab);
+  };
+#define SHP_SYN(B, ST)                                                     \
+  do {                                                                     \
+    if (a.dp) go(k_phase2ring_fft<S, B, ST, double>);                      \
+    else go(k_phase2ring_fft<S, B, ST, float>);                            \
+  } while (0)
   if (synth) {
-    if (blue) {
-      if (a.dp) hipLaunchKernelGGL((k_phase2ring_fft<S, true, double>), grid, block, 0, st, a, plans, cpb);
-      else hipLaunchKernelGGL((k_phase2ring_fft<S, true, float>), grid, block, 0, st, a, plans, cpb);
-    } else {
-      if (a.dp) hipLaunchKernelGGL((k_phase2ring_fft<S, false, double>), grid, block, 0, st, a, plans, cpb);
-      else hipLaunchKernelGGL((k_phase2ring_fft<S, false, float>), grid, block, 0, st, a, plans, cpb);
-    }
+    if (blue) { if (staged) SHP_SYN(true, true); else SHP_SYN(true, false); }
+    else { if (staged) SHP_SYN(false, true); else SHP_SYN(false, false); }
   } else {
     if (blue) {
-      if (a.dp) hipLaunchKernelGGL((k_ring2phase_fft<S, true, double>), grid, block, 0, st, a, plans, cpb);
-      else hipLaunchKernelGGL((k_ring2phase_fft<S, true, float>), grid, block, 0, st, a, plans, cpb);
+      if (a.dp) go(k_ring2phase_fft<S, true, double>);
+      else go(k_ring2phase_fft<S, true, float>);
     } else {
-      if (a.dp) hipLaunchKernelGGL((k_ring2phase_fft<S, false, double>), grid, block, 0, st, a, plans, cpb);
-      else hipLaunchKernelGGL((k_ring2phase_fft<S, false, float>), grid, block, 0, st, a, plans, cpb);
+      if (a.dp) go(k_ring2phase_fft<S, false, double>);
+      else go(k_ring2phase_fft<S, false, float>);
     }
   }
+#undef SHP_SYN
   check_launch(synth ? "phase2ring_fft" : "ring2phase_fft");
+}
+
+template <int S>
+static void launch_class(bool synth, bool blue, const RingArgs &a, const RingFftPlan *plans,
+                         const RingFftPlan *hplans, int nplans, const double2 *btab,
+                         hipStream_t st)
+{
+  if (!synth) {
+    launch_range<S>(false, blue, false, a, plans, hplans, nplans, btab, st);
+    return;
+  }
+  // plans are sorted by decreasing n: rings with n > mmax fold directly
+  int k = 0;
+  while (k < nplans && hplans[k].n > a.mmax) ++k;
+  launch_range<S>(true, blue, false, a, plans, hplans, k, btab, st);
+  launch_range<S>(true, blue, true, a, plans + k, hplans + k, nplans - k, btab, st);
 }
 
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st)
@@ -536,16 +666,46 @@ void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStr
     for (int blue = 1; blue >= 0; --blue) {
       const int idx = 2 * c + blue;
       if (fc.count[idx] == 0) continue;
-      const RingFftPlan *p = fc.plans[idx];
+      const RingFftPlan *p = fc.plans[idx], *hp = fc.hplans[idx];
       const int np = fc.count[idx];
       switch (c) {
-      case 0: launch_class<512>(synth, blue, a, p, np, st); break;
-      case 1: launch_class<1024>(synth, blue, a, p, np, st); break;
-      case 2: launch_class<2048>(synth, blue, a, p, np, st); break;
-      case 3: launch_class<4096>(synth, blue, a, p, np, st); break;
-      case 4: launch_class<8192>(synth, blue, a, p, np, st); break;
+      case 0: launch_class<512>(synth, blue, a, p, hp, np, fc.btab, st); break;
+      case 1: launch_class<1024>(synth, blue, a, p, hp, np, fc.btab, st); break;
+      case 2: launch_class<2048>(synth, blue, a, p, hp, np, fc.btab, st); break;
+      case 3: launch_class<4096>(synth, blue, a, p, hp, np, fc.btab, st); break;
+      case 4: launch_class<8192>(synth, blue, a, p, hp, np, fc.btab, st); break;
       default: break;
       }
+    }
+  }
+}
+
+template <int S>
+static void table_class(const RingFftPlan *plans, const RingFftPlan *hplans, int np,
+                        double2 *btab, hipStream_t st)
+{
+  if (np <= 0) return;
+  size_t need = 0;
+  for (int i = 0; i < np; ++i) need = std::max(need, plan_need(hplans[i]));
+  const size_t lds = need * sizeof(double2);
+  set_lds(k_bluestein_table<S>, lds);
+  hipLaunchKernelGGL(k_bluestein_table<S>, dim3(np), dim3(kThreads<S>), lds, st, plans, btab);
+  check_launch("bluestein_table");
+}
+
+void launch_bluestein_table(const FftClasses &u, double2 *btab, hipStream_t st)
+{
+  for (int c = 0; c < kNumFftClasses; ++c) {
+    const int idx = 2 * c + 1;
+    const RingFftPlan *p = u.plans[idx], *hp = u.hplans[idx];
+    const int np = u.count[idx];
+    switch (c) {
+    case 0: table_class<512>(p, hp, np, btab, st); break;
+    case 1: table_class<1024>(p, hp, np, btab, st); break;
+    case 2: table_class<2048>(p, hp, np, btab, st); break;
+    case 3: table_class<4096>(p, hp, np, btab, st); break;
+    case 4: table_class<8192>(p, hp, np, btab, st); break;
+    default: break;
     }
   }
 }

@@ -249,6 +249,30 @@ def execute_device(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags,
     return op.value
 
 
+def _dev_call(fn, spin, alms, maps, geom, ainfo, ntrans, flags, stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+    fn(spin, ap, mp, geom.ptr, ainfo.ptr, ntrans, flags, ct.c_void_p(stream.cuda_stream))
+
+
+def alm2map_adjoint_device(spin, alms, maps, geom, ainfo, ntrans, flags, stream=None):
+    """Adjoint of synthesis on device tensors: alms = Y^T maps (map2alm without
+    quadrature weights, libsharp's SHARP_Yt).  alms are overwritten (or
+    accumulated with ADD)."""
+    _dev_call(LIB.sharp_hip_alm2map_adjoint_device, spin, alms, maps, geom, ainfo,
+              ntrans, flags, stream)
+
+
+def map2alm_adjoint_device(spin, alms, maps, geom, ainfo, ntrans, flags, stream=None):
+    """Adjoint of analysis on device tensors: maps = W Y alms (alm2map with
+    quadrature weights, libsharp's SHARP_WY)."""
+    _dev_call(LIB.sharp_hip_map2alm_adjoint_device, spin, alms, maps, geom, ainfo,
+              ntrans, flags, stream)
+
+
 class SharpJob:
     """Mirror of sharp_cxxjob<T> (libsharp/sharp_cxx.h:39-149) on numpy arrays;
     dtype float64 (SHARP_DP) or float32."""

@@ -1,0 +1,50 @@
+"""Device-resident entry points (include/sharp_hip.h): stream semantics and
+the profiling interface.
+
+NULL is the null stream: work on PyTorch's default stream (whose handle is
+0) before and after a call is ordered with it, with no explicit
+synchronisation.  A fill of the outputs with NaN right before the call, and
+a reduction right after it, must both see the transform in order."""
+import numpy as np
+import pytest
+import torch
+
+import sharp_hip as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_default_stream_ordering(gpu, spin):
+    nside, lmax = 256, 512
+    geom, ainfo = S.Geometry.healpix(nside), S.AlmInfo.triangular(lmax)
+    nalm = S.nalm_triangular(lmax)
+    nc = 1 if spin == 0 else 2
+    rng = np.random.default_rng(5)
+    alms = [torch.tensor(rng.normal(size=nalm) + 1j * rng.normal(size=nalm), device=gpu)
+            for _ in range(nc)]
+    ref = [torch.zeros(geom.npix, dtype=torch.float64, device=gpu) for _ in range(nc)]
+    S.execute_device(S.ALM2MAP, spin, alms, ref, geom, ainfo, 1, S.DP)
+    torch.cuda.synchronize()
+    want = [float(r.sum()) for r in ref]
+    for _ in range(3):
+        out = [torch.full((geom.npix,), float("nan"), dtype=torch.float64, device=gpu)
+               for _ in range(nc)]
+        S.execute_device(S.ALM2MAP, spin, alms, out, geom, ainfo, 1, S.DP)
+        got = [float(o.sum()) for o in out]   # queued after the transform
+        assert got == want, (got, want)
+
+
+def test_profile_stats(gpu):
+    geom, ainfo = S.Geometry.healpix(32), S.AlmInfo.triangular(64)
+    nalm = S.nalm_triangular(64)
+    alm = [torch.ones(nalm, dtype=torch.complex128, device=gpu)]
+    m = [torch.zeros(geom.npix, dtype=torch.float64, device=gpu)]
+    S.profile(True)
+    S.execute_device(S.ALM2MAP, 0, alm, m, geom, ainfo, 1, S.DP)
+    S.execute_device(S.MAP2ALM, 0, alm, m, geom, ainfo, 1, S.DP)
+    stats = {st["name"]: st for st in S.profile_stats()}
+    S.profile(False)
+    assert "legendre_synth_spin0" in stats and "legendre_anal_spin0" in stats
+    assert stats["legendre_synth_spin0"]["flops"] > 0
+    assert all(st["ms"] >= 0 and st["launches"] >= 1 for st in stats.values())
