@@ -45,6 +45,42 @@ void sharp_hip_map2alm_adjoint_device (int spin, void *alm, void *map,
   const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
   int ntrans, int flags, void *stream);
 
+/* Batches of transforms of uneven size (several geometries, lmax, spins and
+   transform counts), no counterpart in the reference (whose callers issue one
+   sharp_execute per geometry).  `alm`/`map` are host arrays of device
+   pointers as in sharp_execute.  All jobs of a batch are syntheses
+   (SHARP_ALM2MAP, SHARP_WY, SHARP_ALM2MAP_DERIV1) or all analyses
+   (SHARP_MAP2ALM, SHARP_Yt), with the same `flags`.  With
+   SHARP_HIP_MIXED_FP32 the Legendre stage of all jobs runs as one persistent
+   launch per kernel kind (spin 0, spin s, deriv1), with the transform group
+   size and the workgroups per CU chosen by timing the batch at creation
+   (cached by batch shape; SHARP_HIP_TUNE=0 disables, SHARP_HIP_TUNE_CACHE
+   names a file that keeps the choices).  With use_graph the whole batch is
+   recorded into a HIP graph at creation and each execute replays it: the
+   device buffers of the jobs must stay the same (their contents may
+   change). */
+typedef struct {
+  int type;                         /* a sharp_jobtype */
+  int spin;
+  void *alm;                        /* host array of device pointers */
+  void *map;                        /* host array of device pointers */
+  const sharp_geom_info *geom_info;
+  const sharp_alm_info *alm_info;
+  int ntrans;
+} sharp_hip_job;
+
+typedef struct sharp_hip_batch sharp_hip_batch;
+
+sharp_hip_batch *sharp_hip_batch_create (int njobs, const sharp_hip_job *jobs,
+  int flags, int use_graph);
+/* Enqueues the batch on `stream` (NULL: the null stream). */
+void sharp_hip_batch_execute (sharp_hip_batch *batch, void *stream);
+/* The tuned configuration: transforms per tile, persistent workgroups per
+   CU, and whether a graph is replayed. */
+void sharp_hip_batch_config (const sharp_hip_batch *batch, int *tgroup,
+  int *wg_per_cu, int *graph);
+void sharp_hip_batch_destroy (sharp_hip_batch *batch);
+
 /* Builds (and caches) the device tables for this geometry / a_lm layout /
    spin outside any timed region.  Optional: the first transform does it. */
 void sharp_hip_prepare (const sharp_geom_info *geom_info,

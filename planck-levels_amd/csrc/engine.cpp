@@ -661,27 +661,56 @@ static void timed(Ctx &c, hipStream_t st, const char *name, double flops,
 // ---------------------------------------------------------------------------
 // device pipeline
 // ---------------------------------------------------------------------------
-static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
-                       int64_t alm_base, int64_t map_base, hipStream_t st,
-                       unsigned long long *opcnt)
+// Everything one job needs on the device: its plans, the launch arguments of
+// every stage and the algorithmic work of each stage (for the profiler).
+struct JobState {
+  Job j;
+  GeomPlan *gp = nullptr;
+  GeomPlan::Mlim *ml = nullptr;
+  AlmArgs aa{};
+  RingArgs ra{};
+  LegArgs la{};
+  FftClasses fc{};
+  int ncomp_alm = 0, ncomp_map = 0;
+  double triples = 0., c0 = 0., c1 = 0.;
+  double ring_bytes_fft = 0., ring_bytes_all = 0., ring_bytes_dft = 0., alm_bytes = 0.;
+};
+
+static size_t phase_elems(const Job &j, const GeomPlan &gp)
 {
+  const int nmb = (j.mmax + MB) / MB;
+  return (size_t)j.ntrans * j.nmaps * nmb * gp.nslots * 2 * MB;
+}
+
+// kAlmSlackRows rows of slack: the mixed synthesis kernels read up to 15 rows
+// past lmax (masked) without bounds checks
+static size_t almtmp_elems(const Job &j)
+{
+  return ((size_t)tri_ofs(j.mmax + 1, j.lmax) + kAlmSlackRows) * j.ntrans * j.nalm;
+}
+
+static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
+                          int64_t alm_base, int64_t map_base, double2 *phase,
+                          double2 *almtmp, unsigned long long *opcnt)
+{
+  JobState s;
+  s.j = j;
   GeomPlan &gp = get_geom(c, j.g);
   GeomPlan::Mlim &ml = get_mlim(gp, j.lmax, j.spin);
   YlmPlan &yp = get_ylm(c, j.lmax, j.mmax, j.spin);
   double *d_norm = get_norm(c, j.lmax, j.spin, j.deriv);
   int64_t *d_mvs = get_almplan(c, j.a, j.mmax);
-  if (opcnt) *opcnt = op_count(j, gp, ml);
+  s.gp = &gp;
+  s.ml = &ml;
+  const unsigned long long ops = op_count(j, gp, ml);
+  if (opcnt) *opcnt = ops;
 
   const int ncomp_alm = j.ntrans * j.nalm, ncomp_map = j.ntrans * j.nmaps;
+  s.ncomp_alm = ncomp_alm;
+  s.ncomp_map = ncomp_map;
   const int nmb = (j.mmax + MB) / MB;
-  const size_t nphase = (size_t)ncomp_map * nmb * gp.nslots * 2 * MB;
-  double2 *phase = static_cast<double2 *>(c.phase.get(nphase * sizeof(double2)));
-  // kAlmSlackRows rows of slack: the mixed synthesis kernels read up to 15
-  // rows past lmax (masked) without bounds checks
-  const size_t nalmt = ((size_t)tri_ofs(j.mmax + 1, j.lmax) + kAlmSlackRows) * ncomp_alm;
-  double2 *almtmp = static_cast<double2 *>(c.almtmp.get(nalmt * sizeof(double2)));
 
-  AlmArgs aa{};
+  AlmArgs &aa = s.aa;
   aa.lmax = j.lmax;
   aa.mmax = j.mmax;
   aa.ncomp = ncomp_alm;
@@ -699,7 +728,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   aa.c0 = 0;
   aa.npc = ncomp_alm;
 
-  RingArgs ra{};
+  RingArgs &ra = s.ra;
   ra.nslots = gp.nslots;
   ra.nmb = nmb;
   ra.mmax = j.mmax;
@@ -713,7 +742,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   ra.max_lds = c.max_lds;
   for (int i = 0; i < ncomp_map; ++i) ra.maps.p[i] = dmap[i];
 
-  LegArgs la{};
+  LegArgs &la = s.la;
   la.lmax = j.lmax;
   la.mmax = j.mmax;
   la.nslots = gp.nslots;
@@ -733,86 +762,127 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   la.alm = almtmp;
   la.almout = almtmp;
   la.phase = phase;
+  la.ngroups = (int)ml.hgmax.size();
+  la.gmax = ml.gmax;
 
-  const int gmax = (j.spin == 0) ? 4 : 2;
   // triples (m, ring pair, l) with m <= mlim of the pair, and the flop model
   // constants of libsharp/sharp_core_inc2.c:171-173,440-442,583-585
-  const double triples = (double)op_count(j, gp, ml) /
-      (double)(j.deriv ? 12 + 8 * j.ntrans : (j.spin == 0 ? 4 + 4 * j.ntrans : 12 + 16 * j.ntrans));
-  const double c0 = j.spin == 0 ? 4. : 12., c1 = j.deriv ? 8. : (j.spin == 0 ? 4. : 16.);
-  const char *syn_name = j.deriv ? "legendre_synth_deriv1"
-                                 : (j.spin == 0 ? "legendre_synth_spin0" : "legendre_synth_spin");
-  const char *ana_name = j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin";
-  auto for_groups = [&](const char *name, auto &&fn) {
-    int t = 0;
-    while (t < j.ntrans) {
-      int g = gmax;
-      while (g > j.ntrans - t) g >>= 1;
-      la.t0 = t;
-      timed(c, st, name, triples * (c0 + c1 * g), 0., [&] { fn(g); });
-      t += g;
-    }
-  };
+  s.c0 = j.spin == 0 ? 4. : 12.;
+  s.c1 = j.deriv ? 8. : (j.spin == 0 ? 4. : 16.);
+  s.triples = (double)ops / (s.c0 + s.c1 * j.ntrans);
 
-  FftClasses fc{};
-  int64_t nring_fft = 0, npix_fft = 0, npix_all = 0;
-  fc.btab = gp.btab;
+  int64_t nring_fft = 0;
+  s.fc.btab = gp.btab;
   for (int cl = 0; cl < 2 * kNumFftClasses; ++cl) {
-    fc.plans[cl] = gp.fft[cl];
-    fc.hplans[cl] = gp.hfft[cl].data();
-    fc.count[cl] = gp.nfft[cl];
+    s.fc.plans[cl] = gp.fft[cl];
+    s.fc.hplans[cl] = gp.hfft[cl].data();
+    s.fc.count[cl] = gp.nfft[cl];
     nring_fft += gp.nfft[cl];
   }
-  npix_fft = gp.npix_fft;
-  npix_all = gp.npix_all;
-  const bool nofft = (j.flags & SHARP_NO_FFT) != 0;
-  // SHARP_HIP_RING_DFT=1 routes every ring through the direct-summation
-  // kernels (a cross-check of the FFT path; slow for long rings)
-  static const bool force_dft = [] {
+  const double esz = (j.flags & SHARP_DP) ? 8. : 4.;
+  // compulsory traffic of the ring stage: the phases of every ring and its pixels
+  s.ring_bytes_fft =
+      (double)ncomp_map * ((double)nring_fft * (j.mmax + 1) * 16. + (double)gp.npix_fft * esz);
+  s.ring_bytes_all =
+      (double)ncomp_map * ((double)gp.nall * (j.mmax + 1) * 16. + (double)gp.npix_all * esz);
+  s.ring_bytes_dft = std::max(0., s.ring_bytes_all - s.ring_bytes_fft);
+  s.alm_bytes = (double)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm * (16. + 2. * esz);
+  return s;
+}
+
+// SHARP_HIP_RING_DFT=1 routes every ring through the direct-summation kernels
+// (a cross-check of the FFT path; slow for long rings)
+static bool force_ring_dft()
+{
+  static const bool f = [] {
     const char *e = std::getenv("SHARP_HIP_RING_DFT");
     return e && e[0] == '1';
   }();
-  const double esz = (j.flags & SHARP_DP) ? 8. : 4.;
-  // compulsory traffic of the ring stage: the phases of every ring and its pixels
-  const double ring_bytes_fft =
-      (double)ncomp_map * ((double)nring_fft * (j.mmax + 1) * 16. + (double)npix_fft * esz);
-  const double ring_bytes_all =
-      (double)ncomp_map * ((double)gp.nall * (j.mmax + 1) * 16. + (double)npix_all * esz);
-  const double ring_bytes_dft = std::max(0., ring_bytes_all - ring_bytes_fft);
-  auto rings_to_phase = [&]() {
-    if (nofft || force_dft) {
-      timed(c, st, "ring_dft_anal", 0., ring_bytes_all,
-            [&] { launch_ring2phase_dft(ra, gp.all_list, gp.nall, st); });
-    } else {
-      timed(c, st, "ring_fft_anal", 0., ring_bytes_fft, [&] { launch_ring_fft(false, ra, fc, st); });
-      if (gp.ndft)
-        timed(c, st, "ring_dft_anal", 0., ring_bytes_dft,
-              [&] { launch_ring2phase_dft(ra, gp.dft_list, gp.ndft, st); });
-    }
-  };
-  auto phase_to_rings = [&]() {
-    if (nofft || force_dft) {
-      timed(c, st, "ring_dft_synth", 0., ring_bytes_all,
-            [&] { launch_phase2ring_dft(ra, gp.all_list, gp.nall, st); });
-    } else {
-      timed(c, st, "ring_fft_synth", 0., ring_bytes_fft, [&] { launch_ring_fft(true, ra, fc, st); });
-      if (gp.ndft)
-        timed(c, st, "ring_dft_synth", 0., ring_bytes_dft,
-              [&] { launch_phase2ring_dft(ra, gp.dft_list, gp.ndft, st); });
-    }
-  };
-  const double alm_bytes = (double)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm * (16. + 2. * esz);
+  return f;
+}
+
+static void stage_rings_to_phase(Ctx &c, JobState &s, hipStream_t st)
+{
+  const GeomPlan &gp = *s.gp;
+  if ((s.j.flags & SHARP_NO_FFT) || force_ring_dft()) {
+    timed(c, st, "ring_dft_anal", 0., s.ring_bytes_all,
+          [&] { launch_ring2phase_dft(s.ra, gp.all_list, gp.nall, st); });
+  } else {
+    timed(c, st, "ring_fft_anal", 0., s.ring_bytes_fft,
+          [&] { launch_ring_fft(false, s.ra, s.fc, st); });
+    if (gp.ndft)
+      timed(c, st, "ring_dft_anal", 0., s.ring_bytes_dft,
+            [&] { launch_ring2phase_dft(s.ra, gp.dft_list, gp.ndft, st); });
+  }
+}
+
+static void stage_phase_to_rings(Ctx &c, JobState &s, hipStream_t st)
+{
+  const GeomPlan &gp = *s.gp;
+  if ((s.j.flags & SHARP_NO_FFT) || force_ring_dft()) {
+    timed(c, st, "ring_dft_synth", 0., s.ring_bytes_all,
+          [&] { launch_phase2ring_dft(s.ra, gp.all_list, gp.nall, st); });
+  } else {
+    timed(c, st, "ring_fft_synth", 0., s.ring_bytes_fft,
+          [&] { launch_ring_fft(true, s.ra, s.fc, st); });
+    if (gp.ndft)
+      timed(c, st, "ring_dft_synth", 0., s.ring_bytes_dft,
+            [&] { launch_phase2ring_dft(s.ra, gp.dft_list, gp.ndft, st); });
+  }
+}
+
+static void stage_gather(Ctx &c, JobState &s, hipStream_t st)
+{
+  timed(c, st, "alm_gather", 0., s.alm_bytes, [&] { launch_alm_gather(s.aa, st); });
+}
+
+// FP64 Legendre stage of one job
+static void stage_legendre_fp64(Ctx &c, JobState &s, hipStream_t st)
+{
+  const Job &j = s.j;
+  const int gmax = (j.spin == 0) ? 4 : 2;
+  const char *syn_name = j.deriv ? "legendre_synth_deriv1"
+                                 : (j.spin == 0 ? "legendre_synth_spin0" : "legendre_synth_spin");
+  const char *ana_name = j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin";
+  int t = 0;
+  while (t < j.ntrans) {
+    int g = gmax;
+    while (g > j.ntrans - t) g >>= 1;
+    s.la.t0 = t;
+    if (j.type == SHARP_ALM2MAP)
+      timed(c, st, syn_name, s.triples * (s.c0 + s.c1 * g), 0., [&] { launch_synth(s.la, g, st); });
+    else
+      timed(c, st, ana_name, s.triples * (s.c0 + s.c1 * g), 0., [&] { launch_anal(s.la, g, st); });
+    t += g;
+  }
+}
+
+static const char *mixed_name(const Job &j)
+{
+  if (j.type == SHARP_ALM2MAP)
+    return j.deriv ? "mixed_synth_deriv1" : (j.spin == 0 ? "mixed_synth_spin0" : "mixed_synth_spin");
+  return j.spin == 0 ? "mixed_anal_spin0" : "mixed_anal_spin";
+}
+
+static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
+                       int64_t alm_base, int64_t map_base, hipStream_t st,
+                       unsigned long long *opcnt)
+{
+  GeomPlan &gp = get_geom(c, j.g);
+  double2 *phase = static_cast<double2 *>(c.phase.get(phase_elems(j, gp) * sizeof(double2)));
+  double2 *almtmp = static_cast<double2 *>(c.almtmp.get(almtmp_elems(j) * sizeof(double2)));
+  JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt);
 
   const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
   if (!mixed) {
     if (j.type == SHARP_ALM2MAP) {
-      timed(c, st, "alm_gather", 0., alm_bytes, [&] { launch_alm_gather(aa, st); });
-      for_groups(syn_name, [&](int g) { launch_synth(la, g, st); });
-      phase_to_rings();
+      stage_gather(c, s, st);
+      stage_legendre_fp64(c, s, st);
+      stage_phase_to_rings(c, s, st);
     } else {
-      rings_to_phase();
-      for_groups(ana_name, [&](int g) { launch_anal(la, g, st); });
-      timed(c, st, "alm_scatter", 0., alm_bytes, [&] { launch_alm_scatter(aa, st); });
+      stage_rings_to_phase(c, s, st);
+      stage_legendre_fp64(c, s, st);
+      timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
     }
     return;
   }
@@ -820,35 +890,36 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   // mixed precision: transforms in groups of up to 16 (spin 0) or 8 (spin s)
   // per launch, i.e. up to 32 real MFMA columns
   const int tmax = (j.spin == 0) ? kMixMaxT0 : kMixMaxTS;
-  const char *msyn = j.deriv ? "mixed_synth_deriv1"
-                             : (j.spin == 0 ? "mixed_synth_spin0" : "mixed_synth_spin");
-  const char *mana = j.spin == 0 ? "mixed_anal_spin0" : "mixed_anal_spin";
+  LegArgs &la = s.la;
   if (j.type == SHARP_ALM2MAP) {
-    timed(c, st, "alm_gather", 0., alm_bytes, [&] { launch_alm_gather(aa, st); });
+    stage_gather(c, s, st);
     for (int t = 0; t < j.ntrans; t += tmax) {
       la.t0 = t;
       la.ntl = std::min(tmax, j.ntrans - t);
-      timed(c, st, msyn, triples * (c0 + c1 * la.ntl), 0., [&] { launch_msynth(la, st); });
+      timed(c, st, mixed_name(j), s.triples * (s.c0 + s.c1 * la.ntl), 0.,
+            [&] { launch_msynth(la, st); });
     }
-    phase_to_rings();
+    stage_phase_to_rings(c, s, st);
   } else {
-    rings_to_phase();
+    stage_rings_to_phase(c, s, st);
     const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
-    const int ng = (int)ml.hgmax.size();
+    const int ng = (int)s.ml->hgmax.size();
     for (int t = 0; t < j.ntrans; t += tmax) {
       la.t0 = t;
       la.ntl = std::min(tmax, j.ntrans - t);
       const int npc = la.ntl * j.nalm;
-      la.ngroups = ng;
-      la.gmax = ml.gmax;
-      la.part = static_cast<float *>(
-          c.part.get((size_t)ng * nrows * 2 * npc * sizeof(float)));
-      timed(c, st, mana, triples * (c0 + c1 * la.ntl), 0., [&] { launch_manal(la, st); });
-      aa.part = la.part;
-      aa.c0 = t * j.nalm;
-      aa.npc = npc;
-      timed(c, st, "alm_scatter", 0., alm_bytes * npc / ncomp_alm,
-            [&] { launch_alm_scatter(aa, st); });
+      la.part = static_cast<float *>(c.part.get((size_t)ng * nrows * 2 * npc * sizeof(float)));
+      la.part_ld = 2 * npc;
+      la.part_col0 = 0;
+      timed(c, st, mixed_name(j), s.triples * (s.c0 + s.c1 * la.ntl), 0.,
+            [&] { launch_manal(la, st); });
+      s.aa.part = la.part;
+      s.aa.part_ld = la.part_ld;
+      s.aa.part_c0 = t * j.nalm;
+      s.aa.c0 = t * j.nalm;
+      s.aa.npc = npc;
+      timed(c, st, "alm_scatter", 0., s.alm_bytes * npc / s.ncomp_alm,
+            [&] { launch_alm_scatter(s.aa, st); });
     }
   }
 }
@@ -884,6 +955,281 @@ static void map_extent(const sharp_geom_info *g, int64_t &lo, int64_t &hi)
       lo = std::min(lo, std::min(e0, e1));
       hi = std::max(hi, std::max(e0, e1));
     }
+}
+
+// ---------------------------------------------------------------------------
+// batches of transforms of uneven size: one persistent launch per Legendre
+// kernel kind over the tiles of every job, recorded into a HIP graph
+// ---------------------------------------------------------------------------
+struct BatchWork {
+  int kind = 0, ncb = 1, grid = 0, nitems = 0;
+  BatchItem *items = nullptr;
+  int *xofs = nullptr;
+  double flops = 0.;
+  const char *name = "";
+};
+
+struct BatchPlan {
+  int device = 0;
+  bool synth = true, mixed = true;
+  std::vector<Job> jobs;
+  std::vector<std::vector<void *>> almp, mapp;
+  std::vector<DevBuf> phase, almtmp, part;
+  std::vector<JobState> st;
+  LegArgs *djobs = nullptr;
+  std::vector<BatchWork> work;
+  int tgroup_small = 1;   // tuned: 0 -> small transform groups, 1 -> large
+  int wg_per_cu = 2;      // tuned: persistent workgroups per CU
+  bool use_graph = true;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void free_work()
+  {
+    for (auto &w : work) { (void)hipFree(w.items); (void)hipFree(w.xofs); }
+    work.clear();
+  }
+  ~BatchPlan()
+  {
+    free_work();
+    (void)hipFree(djobs);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    for (auto &b : phase) b.release();
+    for (auto &b : almtmp) b.release();
+    for (auto &b : part) b.release();
+  }
+};
+
+static int batch_kind(const Job &j)
+{
+  if (j.type == SHARP_ALM2MAP) return j.deriv ? kBatchSynthD : (j.spin == 0 ? kBatchSynth0 : kBatchSynthS);
+  return j.spin == 0 ? kBatchAnal0 : kBatchAnalS;
+}
+
+static const char *batch_name(int kind)
+{
+  switch (kind) {
+  case kBatchSynth0: return "batch_synth_spin0";
+  case kBatchSynthS: return "batch_synth_spin";
+  case kBatchSynthD: return "batch_synth_deriv1";
+  case kBatchAnal0: return "batch_anal_spin0";
+  default: return "batch_anal_spin";
+  }
+}
+
+// Work lists for the tuning choice (transform group size, workgroups per
+// CU): tiles grouped by (job, transform group, m-block or m) so that a
+// group's tiles run on one XCD; groups are dealt to the 8 XCDs largest
+// first onto the least loaded one, and each XCD's list is ordered by cost.
+static void build_work(BatchPlan &b, int ncu)
+{
+  b.free_work();
+  for (int kind = kBatchSynth0; kind <= kBatchAnalS; ++kind) {
+    const bool spin0 = (kind == kBatchSynth0 || kind == kBatchAnal0);
+    const bool synth = kind <= kBatchSynthD;
+    const int tg = spin0 ? (b.tgroup_small ? kMixMaxT0 / 2 : kMixMaxT0)
+                         : (b.tgroup_small ? kMixMaxTS / 2 : kMixMaxTS);
+    struct Group { double cost; std::vector<BatchItem> items; };
+    std::vector<Group> groups;
+    double flops = 0.;
+    for (size_t ji = 0; ji < b.jobs.size(); ++ji) {
+      const Job &j = b.jobs[ji];
+      if (batch_kind(j) != kind) continue;
+      const JobState &s = b.st[ji];
+      flops += s.triples * (s.c0 + s.c1 * j.ntrans);
+      const int nslots = s.gp->nslots;
+      for (int t0 = 0; t0 < j.ntrans; t0 += tg) {
+        const int ntl = std::min(tg, j.ntrans - t0);
+        if (synth) {
+          const int nmb = (j.mmax + MB) / MB, nsb = (nslots + 63) / 64;
+          for (int mb = 0; mb < nmb; ++mb) {
+            Group g;
+            g.cost = (double)(j.lmax - mb * MB + 1) * nsb * (s.c0 + s.c1 * ntl);
+            for (int sb = 0; sb < nsb; ++sb)
+              g.items.push_back(BatchItem{(int)ji, t0, ntl, 0, mb, sb, 0, 0});
+            groups.push_back(std::move(g));
+          }
+        } else {
+          const auto &gm = s.ml->hgmax;
+          for (int m = 0; m <= j.mmax; ++m) {
+            Group g;
+            for (int gi = 0; gi < (int)gm.size(); ++gi)
+              if (gm[gi] >= m)
+                g.items.push_back(BatchItem{(int)ji, t0, ntl, 2 * t0 * j.nalm, m, gi, 0, 0});
+            if (g.items.empty()) continue;
+            g.cost = (double)(j.lmax - m + 1) * g.items.size() * (s.c0 + s.c1 * ntl);
+            groups.push_back(std::move(g));
+          }
+        }
+      }
+    }
+    if (groups.empty()) continue;
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const Group &x, const Group &y) { return x.cost > y.cost; });
+    std::vector<BatchItem> lists[8];
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (auto &g : groups) {
+      const int x = (int)(std::min_element(load, load + 8) - load);
+      load[x] += g.cost;
+      lists[x].insert(lists[x].end(), g.items.begin(), g.items.end());
+    }
+    std::vector<BatchItem> all;
+    std::vector<int> xofs(9, 0);
+    for (int x = 0; x < 8; ++x) {
+      xofs[x] = (int)all.size();
+      all.insert(all.end(), lists[x].begin(), lists[x].end());
+    }
+    xofs[8] = (int)all.size();
+    BatchWork w;
+    w.kind = kind;
+    w.ncb = (tg * (spin0 ? 2 : 4) > 16) ? 2 : 1;
+    w.nitems = (int)all.size();
+    w.grid = std::max(8, ((b.wg_per_cu * ncu + 7) / 8) * 8);
+    w.items = upload(all);
+    w.xofs = upload(xofs);
+    w.flops = flops;
+    w.name = batch_name(kind);
+    b.work.push_back(w);
+  }
+}
+
+static void batch_run(Ctx &c, BatchPlan &b, hipStream_t st)
+{
+  const size_t n = b.jobs.size();
+  if (!b.mixed) {
+    // FP64: the jobs one after another
+    for (size_t i = 0; i < n; ++i) {
+      JobState &s = b.st[i];
+      if (b.synth) {
+        stage_gather(c, s, st);
+        stage_legendre_fp64(c, s, st);
+        stage_phase_to_rings(c, s, st);
+      } else {
+        stage_rings_to_phase(c, s, st);
+        stage_legendre_fp64(c, s, st);
+        timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
+      }
+    }
+    return;
+  }
+  if (b.synth) {
+    for (size_t i = 0; i < n; ++i) stage_gather(c, b.st[i], st);
+    for (auto &w : b.work)
+      timed(c, st, w.name, w.flops, 0.,
+            [&] { launch_batch(w.kind, w.ncb, b.djobs, w.items, w.xofs, w.grid, st); });
+    for (size_t i = 0; i < n; ++i) stage_phase_to_rings(c, b.st[i], st);
+  } else {
+    for (size_t i = 0; i < n; ++i) stage_rings_to_phase(c, b.st[i], st);
+    for (auto &w : b.work)
+      timed(c, st, w.name, w.flops, 0.,
+            [&] { launch_batch(w.kind, w.ncb, b.djobs, w.items, w.xofs, w.grid, st); });
+    for (size_t i = 0; i < n; ++i) {
+      JobState &s = b.st[i];
+      timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
+    }
+  }
+}
+
+// key of a batch for the tuning cache
+static std::string batch_key(const BatchPlan &b)
+{
+  std::string k = b.synth ? "S" : "A";
+  char buf[96];
+  for (const Job &j : b.jobs) {
+    const GeomPlan &gp = *b.st[&j - &b.jobs[0]].gp;
+    unsigned long h = 1469598103934665603ul;
+    for (char ch : gp.key) h = (h ^ (unsigned char)ch) * 1099511628211ul;
+    snprintf(buf, sizeof(buf), "|%lx:%d:%d:%d:%d:%d", h, j.lmax, j.mmax, j.spin, j.deriv, j.ntrans);
+    k += buf;
+  }
+  return k;
+}
+
+static std::map<std::string, std::pair<int, int>> g_tune;   // key -> (tgroup_small, wg_per_cu)
+
+static const char *tune_file()
+{
+  const char *f = std::getenv("SHARP_HIP_TUNE_CACHE");
+  return (f && f[0]) ? f : nullptr;
+}
+
+static bool tune_lookup(const std::string &key, std::pair<int, int> &v)
+{
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) { v = it->second; return true; }
+  const char *f = tune_file();
+  if (!f) return false;
+  FILE *fp = fopen(f, "r");
+  if (!fp) return false;
+  std::vector<char> line(1 << 16);
+  bool found = false;
+  while (fgets(line.data(), (int)line.size(), fp)) {
+    int a = 0, w = 0;
+    char *sp = strrchr(line.data(), ' ');
+    if (!sp) continue;
+    *sp = 0;
+    char *sp2 = strrchr(line.data(), ' ');
+    if (!sp2) continue;
+    *sp2 = 0;
+    a = atoi(sp2 + 1);
+    w = atoi(sp + 1);
+    if (key == line.data()) { v = {a, w}; found = true; }
+  }
+  fclose(fp);
+  if (found) g_tune[key] = v;
+  return found;
+}
+
+static void tune_store(const std::string &key, std::pair<int, int> v)
+{
+  g_tune[key] = v;
+  const char *f = tune_file();
+  if (!f) return;
+  if (FILE *fp = fopen(f, "a")) {
+    fprintf(fp, "%s %d %d\n", key.c_str(), v.first, v.second);
+    fclose(fp);
+  }
+}
+
+// Project autotuner: times every (transform group, workgroups per CU) pair
+// on the batch itself (one untimed and one timed run each, HIP events on
+// the plan's stream) and keeps the fastest; the choice is cached by the
+// batch's shape (and in SHARP_HIP_TUNE_CACHE, when set).
+static void batch_tune(Ctx &c, BatchPlan &b, int ncu)
+{
+  std::pair<int, int> best{0, 2};
+  const std::string key = batch_key(b);
+  const char *e = std::getenv("SHARP_HIP_TUNE");
+  const bool off = e && e[0] == '0';
+  if (!tune_lookup(key, best) && !off) {
+    float tbest = 1e30f;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    for (int small = 0; small <= 1; ++small)
+      for (int wpc : {1, 2, 4}) {
+        b.tgroup_small = small;
+        b.wg_per_cu = wpc;
+        build_work(b, ncu);
+        batch_run(c, b, b.cap);
+        HIP_CHECK(hipEventRecord(e0, b.cap));
+        batch_run(c, b, b.cap);
+        HIP_CHECK(hipEventRecord(e1, b.cap));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < tbest) { tbest = ms; best = {small, wpc}; }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_store(key, best);
+  }
+  b.tgroup_small = best.first;
+  b.wg_per_cu = best.second;
+  build_work(b, ncu);
 }
 
 } // namespace shp
@@ -970,6 +1316,114 @@ void sharp_hip_map2alm_adjoint_device(int spin, void *alm, void *map,
 {
   sharp_hip_execute_device(SHARP_WY, spin, alm, map, geom_info, alm_info, ntrans,
                            flags, stream, nullptr);
+}
+
+sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, int flags,
+                                        int use_graph)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (njobs <= 0 || !jobs) SHP_FAIL("empty batch");
+  Ctx &c = context();
+  auto b = std::make_unique<BatchPlan>();
+  b->device = c.device;
+  b->mixed = (flags & SHARP_HIP_MIXED_FP32) != 0;
+  b->use_graph = use_graph != 0;
+  for (int i = 0; i < njobs; ++i) {
+    const sharp_hip_job &q = jobs[i];
+    const Job j = make_job(q.type, q.spin, q.geom_info, q.alm_info, q.ntrans, flags);
+    const bool syn = (j.type == SHARP_ALM2MAP);
+    if (i == 0) b->synth = syn;
+    else if (syn != b->synth) SHP_FAIL("batch jobs must all be syntheses or all analyses");
+    b->jobs.push_back(j);
+    void *const *ap = static_cast<void *const *>(q.alm);
+    void *const *mp = static_cast<void *const *>(q.map);
+    b->almp.emplace_back(ap, ap + (size_t)j.ntrans * j.nalm);
+    b->mapp.emplace_back(mp, mp + (size_t)j.ntrans * j.nmaps);
+  }
+  const size_t n = b->jobs.size();
+  b->phase.resize(n);
+  b->almtmp.resize(n);
+  b->part.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const Job &j = b->jobs[i];
+    GeomPlan &gp = get_geom(c, j.g);
+    double2 *ph = static_cast<double2 *>(b->phase[i].get(phase_elems(j, gp) * sizeof(double2)));
+    double2 *at = static_cast<double2 *>(b->almtmp[i].get(almtmp_elems(j) * sizeof(double2)));
+    b->st.push_back(setup_job(c, j, b->almp[i].data(), b->mapp[i].data(), 0, 0, ph, at, nullptr));
+    JobState &s = b->st.back();
+    if (b->mixed && !b->synth) {
+      // partial sums of every transform of the job: [group][row][2*ncomp_alm]
+      const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
+      const size_t ng = s.ml->hgmax.size();
+      s.la.part = static_cast<float *>(
+          b->part[i].get(ng * (size_t)nrows * 2 * s.ncomp_alm * sizeof(float)));
+      s.la.part_ld = 2 * s.ncomp_alm;
+      s.aa.part = s.la.part;
+      s.aa.part_ld = s.la.part_ld;
+      s.aa.part_c0 = 0;
+      s.aa.c0 = 0;
+      s.aa.npc = s.ncomp_alm;
+    }
+  }
+  std::vector<LegArgs> las;
+  for (auto &s : b->st) las.push_back(s.la);
+  b->djobs = upload(las);
+  HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  int ncu = 256;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess)
+    ncu = 256;
+  const bool prof = c.prof.on;
+  c.prof.on = false;
+  if (b->mixed) batch_tune(c, *b, ncu);
+  batch_run(c, *b, b->cap);   // settles every lazily built plan before capture
+  HIP_CHECK(hipStreamSynchronize(b->cap));
+  if (b->use_graph) {
+    HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    batch_run(c, *b, b->cap);
+    HIP_CHECK(hipStreamEndCapture(b->cap, &b->graph));
+    HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  c.prof.on = prof;
+  return reinterpret_cast<sharp_hip_batch *>(b.release());
+}
+
+void sharp_hip_batch_execute(sharp_hip_batch *batch, void *stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  BatchPlan *b = reinterpret_cast<BatchPlan *>(batch);
+  if (!b) SHP_FAIL("null batch");
+  Ctx &c = context();
+  if (c.device != b->device) SHP_FAIL("batch executed on another device than it was created on");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->exec) {
+    timed(c, st, "batch_graph", 0., 0., [&] { HIP_CHECK(hipGraphLaunch(b->exec, st)); });
+  } else {
+    batch_run(c, *b, st);
+  }
+}
+
+void sharp_hip_batch_config(const sharp_hip_batch *batch, int *tgroup, int *wg_per_cu,
+                            int *graph)
+{
+  const BatchPlan *b = reinterpret_cast<const BatchPlan *>(batch);
+  if (!b) SHP_FAIL("null batch");
+  if (tgroup) {
+    const bool spin0 = !b->jobs.empty() && b->jobs[0].spin == 0;
+    const int big = spin0 ? kMixMaxT0 : kMixMaxTS;
+    *tgroup = b->mixed ? (b->tgroup_small ? big / 2 : big) : 0;
+  }
+  if (wg_per_cu) *wg_per_cu = b->mixed ? b->wg_per_cu : 0;
+  if (graph) *graph = b->exec ? 1 : 0;
+}
+
+void sharp_hip_batch_destroy(sharp_hip_batch *batch)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  BatchPlan *b = reinterpret_cast<BatchPlan *>(batch);
+  if (!b) return;
+  (void)hipStreamSynchronize(b->cap);
+  HIP_CHECK(hipDeviceSynchronize());
+  delete b;
 }
 
 void sharp_hip_prepare(const sharp_geom_info *geom_info,

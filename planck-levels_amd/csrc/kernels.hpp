@@ -28,8 +28,23 @@ struct LegArgs {
   int ntl;              // transforms in this launch
   int ngroups;          // analysis: ring-pair groups per m (partial sums)
   const int *gmax;      // analysis: largest mlim of each group [ngroups]
-  float *part;          // analysis: partial a_lm [ngroups][rows][2*ntl*nalm]
+  float *part;          // analysis: partial a_lm [ngroups][rows][part_ld]
+  int64_t part_ld;      // floats per partial row
+  int part_col0;        // float column of this launch's first transform
 };
+
+// One tile of the persistent batched Legendre kernels: job index, transform
+// group (first transform, count, partial-sum column) and the tile's
+// coordinates (synthesis: m-block, 64-slot block; analysis: m, slot group).
+struct BatchItem {
+  int job, t0, ntl, col0;
+  int major, minor, pad0, pad1;
+};
+enum { kBatchSynth0, kBatchSynthS, kBatchSynthD, kBatchAnal0, kBatchAnalS };
+// grid: number of persistent workgroups (a multiple of 8); xofs[0..8]: the
+// work list's parts for the 8 XCDs
+void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items,
+                  const int *xofs, int grid, hipStream_t st);
 
 // ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st);
@@ -110,10 +125,13 @@ struct AlmArgs {
   PtrTab alms;          // [ncomp] device pointers (float or double data)
   // analysis from partial sums (mixed forms): when part != nullptr the
   // components [c0, c0+npc) are sum_g part[g] over the groups g with
-  // gmax[g] >= m, instead of almtmp
+  // gmax[g] >= m, instead of almtmp; component c of a row is at float
+  // 2*(c - part_c0) of it, rows are part_ld floats apart
   const float *part;
   const int *gmax;
   int ngroups, c0, npc;
+  int64_t part_ld;
+  int part_c0;
 };
 void launch_alm_gather(const AlmArgs &a, hipStream_t st);
 void launch_alm_scatter(const AlmArgs &a, hipStream_t st);

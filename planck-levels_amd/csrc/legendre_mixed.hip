@@ -133,10 +133,8 @@ __device__ inline void store_phases(const LegArgs &a, int m, int sb, int comp0, 
 // two-step of a block, so that the wait for the current coefficients does
 // not also wait for the prefetch.
 template <int NCB>
-__global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
+__device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
 {
-  int mb, sb;
-  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
   const int lane = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int m = mb * MB + w;
@@ -361,10 +359,8 @@ __global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
 // spin s synthesis (and ALM2MAP_DERIV1)
 // ===========================================================================
 template <int NCB, bool DERIV>
-__global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
+__device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 {
-  int mb, sb;
-  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
   const int lane = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int m = mb * MB + w;
@@ -642,8 +638,8 @@ constexpr int kTS = 20;    // LDS row stride of the [16 x 16] reduction tiles
 // Adds the wavefronts' tiles (in slice 0 of each wavefront) in a fixed order
 // and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm.
 template <int NT>
-__device__ inline void reduce_store(float *const *slices, int nw, float *prow, int lc,
-                                    int lmax, int ncol)
+__device__ inline void reduce_store(float *const *slices, int nw, float *prow, int64_t ld,
+                                    int lc, int lmax, int ncol)
 {
   for (int e = threadIdx.x; e < NT * 256; e += blockDim.x) {
     const int tile = e >> 8, i = (e >> 4) & 15, nn = e & 15;
@@ -652,7 +648,7 @@ __device__ inline void reduce_store(float *const *slices, int nw, float *prow, i
     for (int ww = 0; ww < nw; ++ww) sum += slices[ww][off];
     const int x = tile & 1, cb = tile >> 1;
     const int ll = lc + x + 2 * i, col = cb * 16 + nn;
-    if (ll <= lmax && col < ncol) prow[(int64_t)ll * ncol + col] = sum;
+    if (ll <= lmax && col < ncol) prow[(int64_t)ll * ld + col] = sum;
   }
 }
 
@@ -670,13 +666,11 @@ __device__ inline void write_tiles(float *slice, const f4 (&t)[2][NCB])
 }
 
 template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
+__device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
+                                 float (&lds)[kMixWaves0][64 * kLS])
 {
   constexpr int W = kMixWaves0;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
-  __shared__ float lds[W][64 * kLS];
-  int m, g;
-  if (!xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) return;
   if (a.gmax[g] < m) return;   // no ring pair of this group reaches m
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -688,8 +682,8 @@ __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * ncol +
-                (int64_t)(tri_ofs(m, lmax) - m) * ncol;
+  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
+                (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
 
   // B: p1 = N+S and p2 = N-S of ring pairs wbase+4j+kq, column n
   float bp[2][16][NCB];
@@ -811,19 +805,17 @@ __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     write_tiles<NCB>(mine, t);
     __syncthreads();
-    reduce_store<2 * NCB>(slices, W, prow, lc, lmax, ncol);
+    reduce_store<2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
     __syncthreads();
   }
 }
 
 template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
+__device__ void manal_spin_tile(const LegArgs &a, int m, int g,
+                                float (&lds)[kMixWavesS][2][64 * kLS])
 {
   constexpr int W = kMixWavesS;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
-  __shared__ float lds[W][2][64 * kLS];
-  int m, g;
-  if (!xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) return;
   if (a.gmax[g] < m) return;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -836,14 +828,14 @@ __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * ncol +
-                (int64_t)(tri_ofs(m, lmax) - m) * ncol;
+  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
+                (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
 
   // rows m <= l < mhi have no lambda: zero them
   {
     const int nz = ((st.mhi < lmax + 1 ? st.mhi : lmax + 1) - m) * ncol;
     for (int e = threadIdx.x; e < nz; e += blockDim.x)
-      prow[(int64_t)(m + e / ncol) * ncol + e % ncol] = 0.f;
+      prow[(int64_t)(m + e / ncol) * a.part_ld + e % ncol] = 0.f;
   }
   if (st.mhi > lmax) return;
   const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
@@ -990,9 +982,108 @@ __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     write_tiles<NCB>(mw, tw);
     __syncthreads();
-    reduce_store<2 * NCB>(slices, W, prow, lc, lmax, ncol);
+    reduce_store<2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
     __syncthreads();
   }
+}
+
+// ===========================================================================
+// kernels: one job (grid over its tiles), or a batch (persistent grid over a
+// work list of several jobs)
+// ===========================================================================
+template <int NCB>
+__global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin0_tile<NCB>(a, mb, sb);
+}
+
+template <int NCB, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin_tile<NCB, DERIV>(a, mb, sb);
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
+{
+  __shared__ float lds[kMixWaves0][64 * kLS];
+  int m, g;
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin0_tile<NCB>(a, m, g, lds);
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
+{
+  __shared__ float lds[kMixWavesS][2][64 * kLS];
+  int m, g;
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin_tile<NCB>(a, m, g, lds);
+}
+
+// Persistent form: workgroup b works through the items of XCD b%8 (the work
+// list is split into 8 contiguous parts, xofs[0..8]) with a stride of the
+// number of workgroups per XCD, so the tiles of one (job, m-block) stay in
+// one L2.  The grid size is a multiple of 8.
+template <typename F>
+__device__ inline void for_items(const BatchItem *items, const int *xofs, F &&f)
+{
+  const int x = blockIdx.x & 7, nper = gridDim.x >> 3;
+  for (int p = xofs[x] + (blockIdx.x >> 3); p < xofs[x + 1]; p += nper) {
+    const BatchItem it = items[p];
+    f(it);
+  }
+}
+
+__device__ inline LegArgs job_args(const LegArgs *jobs, const BatchItem &it)
+{
+  LegArgs a = jobs[it.job];
+  a.t0 = it.t0;
+  a.ntl = it.ntl;
+  a.part_col0 = it.col0;
+  return a;
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * MB) kb_msynth_spin0(const LegArgs *jobs,
+                                                           const BatchItem *items, const int *xofs)
+{
+  for_items(items, xofs, [&](const BatchItem &it) {
+    msynth_spin0_tile<NCB>(job_args(jobs, it), it.major, it.minor);
+  });
+}
+
+template <int NCB, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *jobs,
+                                                          const BatchItem *items, const int *xofs)
+{
+  for_items(items, xofs, [&](const BatchItem &it) {
+    msynth_spin_tile<NCB, DERIV>(job_args(jobs, it), it.major, it.minor);
+  });
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWaves0) kb_manal_spin0(const LegArgs *jobs,
+                                                                  const BatchItem *items,
+                                                                  const int *xofs)
+{
+  __shared__ float lds[kMixWaves0][64 * kLS];
+  for_items(items, xofs, [&](const BatchItem &it) {
+    manal_spin0_tile<NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    __syncthreads();   // the next item reuses the LDS slices
+  });
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * kMixWavesS) kb_manal_spin(const LegArgs *jobs,
+                                                                 const BatchItem *items,
+                                                                 const int *xofs)
+{
+  __shared__ float lds[kMixWavesS][2][64 * kLS];
+  for_items(items, xofs, [&](const BatchItem &it) {
+    manal_spin_tile<NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    __syncthreads();
+  });
 }
 
 // ===========================================================================
@@ -1028,6 +1119,39 @@ void launch_msynth(const LegArgs &a, hipStream_t st)
     else hipLaunchKernelGGL((k_msynth_spin<1, false>), grid, block, 0, st, a, nsb);
   }
   check_launch("mixed synthesis");
+}
+
+void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items,
+                  const int *xofs, int grid, hipStream_t st)
+{
+  if (grid <= 0) return;
+  if (grid % 8) SHP_FAIL("persistent grid must be a multiple of 8");
+  const dim3 g(grid);
+  const bool two = ncb == 2;
+  switch (kind) {
+  case kBatchSynth0:
+    if (two) hipLaunchKernelGGL(kb_msynth_spin0<2>, g, dim3(64, MB), 0, st, jobs, items, xofs);
+    else hipLaunchKernelGGL(kb_msynth_spin0<1>, g, dim3(64, MB), 0, st, jobs, items, xofs);
+    break;
+  case kBatchSynthS:
+    if (two) hipLaunchKernelGGL((kb_msynth_spin<2, false>), g, dim3(64, MB), 0, st, jobs, items, xofs);
+    else hipLaunchKernelGGL((kb_msynth_spin<1, false>), g, dim3(64, MB), 0, st, jobs, items, xofs);
+    break;
+  case kBatchSynthD:
+    if (two) hipLaunchKernelGGL((kb_msynth_spin<2, true>), g, dim3(64, MB), 0, st, jobs, items, xofs);
+    else hipLaunchKernelGGL((kb_msynth_spin<1, true>), g, dim3(64, MB), 0, st, jobs, items, xofs);
+    break;
+  case kBatchAnal0:
+    if (two) hipLaunchKernelGGL(kb_manal_spin0<2>, g, dim3(64 * kMixWaves0), 0, st, jobs, items, xofs);
+    else hipLaunchKernelGGL(kb_manal_spin0<1>, g, dim3(64 * kMixWaves0), 0, st, jobs, items, xofs);
+    break;
+  case kBatchAnalS:
+    if (two) hipLaunchKernelGGL(kb_manal_spin<2>, g, dim3(64 * kMixWavesS), 0, st, jobs, items, xofs);
+    else hipLaunchKernelGGL(kb_manal_spin<1>, g, dim3(64 * kMixWavesS), 0, st, jobs, items, xofs);
+    break;
+  default: SHP_FAIL("bad batch kernel");
+  }
+  check_launch("batched Legendre stage");
 }
 
 void launch_manal(const LegArgs &a, hipStream_t st)

@@ -284,8 +284,8 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
   const int64_t row = tri_ofs(m, a.lmax) + l - m;
   double2 x;
   if (a.part) {
-    const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * 2 * npc;
-    const float *pp = a.part + row * 2 * npc + 2 * cl;
+    const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * a.part_ld;
+    const float *pp = a.part + row * a.part_ld + 2 * (comp - a.part_c0);
     double re = 0., im = 0.;
     for (int g = 0; g < a.ngroups; ++g) {
       if (a.gmax[g] < m) continue;

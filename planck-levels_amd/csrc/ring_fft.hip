@@ -32,6 +32,7 @@
 #include "devmath.hpp"
 
 #include <algorithm>
+#include <map>
 
 namespace shp {
 
@@ -584,11 +585,15 @@ __global__ void __launch_bounds__(kThreads<S>) k_bluestein_table(const RingFftPl
   for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = smem[j];
 }
 
+// raises a kernel's dynamic LDS limit once per new maximum (callers hold the
+// library's lock)
 template <typename K> static void set_lds(K kern, size_t lds)
 {
-  if (lds > 65536)
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  static std::map<const void *, size_t> done;
+  const void *k = reinterpret_cast<const void *>(kern);
+  if (lds <= 65536 || done[k] >= lds) return;
+  HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  done[k] = lds;
 }
 
 static size_t plan_need(const RingFftPlan &p) { return p.bluestein ? p.M : p.N; }

@@ -98,6 +98,12 @@ def _load():
               "sharp_hip_map2alm_adjoint_device"):
         getattr(lib, f).argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, G, A,
                                     ct.c_int, ct.c_int, ct.c_void_p]
+    lib.sharp_hip_batch_create.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_int]
+    lib.sharp_hip_batch_create.restype = ct.c_void_p
+    lib.sharp_hip_batch_execute.argtypes = [ct.c_void_p, ct.c_void_p]
+    lib.sharp_hip_batch_config.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int),
+                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
+    lib.sharp_hip_batch_destroy.argtypes = [ct.c_void_p]
     lib.sharp_hip_prepare.argtypes = [G, A, ct.c_int]
     lib.sharp_hip_synchronize.argtypes = [ct.c_void_p]
     lib.sharp_hip_device_count.restype = ct.c_int
@@ -271,6 +277,56 @@ def map2alm_adjoint_device(spin, alms, maps, geom, ainfo, ntrans, flags, stream=
     quadrature weights, libsharp's SHARP_WY)."""
     _dev_call(LIB.sharp_hip_map2alm_adjoint_device, spin, alms, maps, geom, ainfo,
               ntrans, flags, stream)
+
+
+class _JobC(ct.Structure):
+    _fields_ = [("type", ct.c_int), ("spin", ct.c_int), ("alm", ct.c_void_p),
+                ("map", ct.c_void_p), ("geom_info", ct.POINTER(GeomInfoC)),
+                ("alm_info", ct.POINTER(AlmInfoC)), ("ntrans", ct.c_int)]
+
+
+class Batch:
+    """A batch of device transforms of uneven size (include/sharp_hip.h,
+    sharp_hip_batch_*): jobs is a list of (jobtype, spin, alms, maps, geom,
+    ainfo, ntrans) with torch CUDA tensors as in execute_device.  The tensors
+    must stay alive (and in place) for the life of the batch; with use_graph
+    each execute() replays the HIP graph recorded at creation."""
+
+    def __init__(self, jobs, flags, use_graph=True):
+        self._keep = []
+        arr = (_JobC * len(jobs))()
+        for k, (jt, spin, alms, maps, geom, ainfo, ntrans) in enumerate(jobs):
+            ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+            mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+            self._keep += [ap, mp, alms, maps, geom, ainfo]
+            arr[k] = _JobC(jt, spin, ct.cast(ap, ct.c_void_p), ct.cast(mp, ct.c_void_p),
+                           geom.ptr, ainfo.ptr, ntrans)
+        self._arr = arr
+        self.ptr = LIB.sharp_hip_batch_create(len(jobs), ct.cast(arr, ct.c_void_p), flags,
+                                              1 if use_graph else 0)
+
+    def execute(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        LIB.sharp_hip_batch_execute(self.ptr, ct.c_void_p(stream.cuda_stream))
+
+    def config(self):
+        t, w, g = ct.c_int(), ct.c_int(), ct.c_int()
+        LIB.sharp_hip_batch_config(self.ptr, ct.byref(t), ct.byref(w), ct.byref(g))
+        return {"transforms_per_tile": t.value, "workgroups_per_cu": w.value,
+                "graph": bool(g.value)}
+
+    def close(self):
+        if getattr(self, "ptr", None) and LIB is not None:
+            LIB.sharp_hip_batch_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SharpJob:
