@@ -144,7 +144,7 @@ def run_form(S, torch, step, warmup, steps):
 
 
 def show(name, stats, ms_per_step, value):
-    log("%s: %.1f ms/step, %.3f TQU round trips/s" % (name, ms_per_step, value))
+    log("%s: %.1f ms/step, %.3f round trips/s" % (name, ms_per_step, value))
     for st in sorted(stats, key=lambda x: -x["ms"]):
         log("  %-24s %9.2f ms  %4d launches  %8.2f TFLOP/s  %8.1f GB/s" % (
             st["name"], st["ms"], st["launches"],
@@ -193,6 +193,86 @@ def roofline(stats, mixed, nt):
             "traffic": None, "avg_launch_ms": avg_s * 1e3}
 
 
+def batch_workload(S, torch, dev, args):
+    """The uneven batch: 24 spin-0 maps each at Nside 128, 256, 512, 1024
+    (lmax = 2 Nside) and 8 Gauss-Legendre grids each at lmax 255 and 767
+    (nlat = lmax+1, nlon = 2 lmax+1); a_lm with C_l = 1/(l(l+1)+1).  One step
+    is alm2map + map2alm of all 112 maps: the mixed form as two batches (one
+    persistent Legendre launch each, replayed HIP graphs), the FP64 form one
+    call per geometry beside it.  The error of every mixed map and round-trip
+    a_lm against the FP64 form is reported."""
+    specs = [("healpix", n, 2 * n, 24) for n in (128, 256, 512, 1024)]
+    specs += [("gauss", 255, 255, 8), ("gauss", 767, 767, 8)]
+    gen = torch.Generator(device=dev).manual_seed(args.seed)
+    jobs = []
+    for kind, n, lmax, nt in specs:
+        geom = S.Geometry.healpix(n) if kind == "healpix" else S.Geometry.iso("gauss", lmax + 1, 2 * lmax + 1)
+        ainfo = S.AlmInfo.triangular(lmax)
+        l_of = torch.cat([torch.arange(m, lmax + 1, device=dev) for m in range(lmax + 1)]).double()
+        sd = torch.sqrt(1.0 / (l_of * (l_of + 1) + 1) / 2)
+        alms = []
+        for _ in range(nt):
+            re = torch.randn(l_of.shape, generator=gen, device=dev, dtype=torch.float64)
+            im = torch.randn(l_of.shape, generator=gen, device=dev, dtype=torch.float64)
+            im[: lmax + 1] = 0.0
+            alms.append(torch.complex(re * sd, im * sd))
+        npix = geom.npix
+        jobs.append({"name": "%s %d lmax %d x%d" % (kind, n, lmax, nt), "geom": geom,
+                     "ainfo": ainfo, "nt": nt, "alms": alms,
+                     "maps": [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(nt)],
+                     "out": [torch.zeros_like(alms[0]) for _ in range(nt)],
+                     "maps64": [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(nt)],
+                     "out64": [torch.zeros_like(alms[0]) for _ in range(nt)]})
+    nmaps = sum(j["nt"] for j in jobs)
+    flags = S.DP | S.MIXED_FP32
+    log("batch: creating plans (autotuning, graph capture)")
+    bs = S.Batch([(S.ALM2MAP, 0, j["alms"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
+                 flags, use_graph=True)
+    ba = S.Batch([(S.MAP2ALM, 0, j["out"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
+                 flags, use_graph=True)
+
+    def step_mixed():
+        bs.execute()
+        ba.execute()
+
+    def step_fp64():
+        for j in jobs:
+            S.execute_device(S.ALM2MAP, 0, j["alms"], j["maps64"], j["geom"], j["ainfo"], j["nt"], S.DP)
+            S.execute_device(S.MAP2ALM, 0, j["out64"], j["maps64"], j["geom"], j["ainfo"], j["nt"], S.DP)
+
+    dt, stats = run_form(S, torch, step_mixed, args.warmup, args.steps)
+    ms = 1e3 * dt / args.steps
+    show("batch mixed", stats, ms, nmaps / (dt / args.steps))
+    dt64, stats64 = run_form(S, torch, step_fp64, args.warmup, args.steps)
+    ms64 = 1e3 * dt64 / args.steps
+    show("per-geometry fp64", stats64, ms64, nmaps / (dt64 / args.steps))
+
+    def rel(x, y):
+        return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+    per = {}
+    for j in jobs:
+        per[j["name"]] = {
+            "maps_max_rel_l2": max(rel(a, b) for a, b in zip(j["maps"], j["maps64"])),
+            "roundtrip_alm_max_rel_l2": max(rel(a, b) for a, b in zip(j["out"], j["out64"]))}
+    log("per-map error vs FP64:", per)
+    out = {
+        "metric": "uneven batch alm2map+map2alm, maps/s (mixed batched form)",
+        "value": nmaps / (dt / args.steps), "unit": "map round trips/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mfma contraction, f64 recurrence/FFT",
+        "data": "synthetic: seeded Gaussian a_lm, C_l=1/(l(l+1)+1)",
+        "config": {"workload": "96 HEALPix maps (24 each Nside 128/256/512/1024, lmax=2 Nside) "
+                               "+ 16 Gauss-Legendre grids (8 each lmax 255/767), spin 0",
+                   "batch_config": {"synthesis": bs.config(), "analysis": ba.config()}},
+        "fp64_per_geometry": {"value": nmaps / (dt64 / args.steps), "ms_per_step": ms64},
+        "error_vs_fp64_per_geometry": per,
+    }
+    print(json.dumps(out), flush=True)
+    bs.close()
+    ba.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +290,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM counters)")
+    ap.add_argument("--workload", default="tqu", choices=["tqu", "batch"],
+                    help="tqu: the headline workload; batch: the uneven batch "
+                         "(96 HEALPix maps of four sizes + 16 Gauss-Legendre grids)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU form of this path "
@@ -222,6 +305,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "batch":
+        batch_workload(S, torch, dev, args)
+        return
     nside = args.nside
     lmax = args.lmax if args.lmax is not None else 2 * nside
     batch = args.batch

@@ -864,6 +864,58 @@ static const char *mixed_name(const Job &j)
   return j.spin == 0 ? "mixed_anal_spin0" : "mixed_anal_spin";
 }
 
+// SHARP_HIP_FP64_ANALYSIS=reduce selects the FP64 analysis kernels that sum
+// over ring pairs with cross-lane reductions (legendre.hip) instead of the
+// FP64 matrix-core kernels; both are exact to FP64 rounding
+static bool fp64_anal_reduce()
+{
+  static const bool f = [] {
+    const char *e = std::getenv("SHARP_HIP_FP64_ANALYSIS");
+    return e && std::strcmp(e, "reduce") == 0;
+  }();
+  return f;
+}
+
+// Analysis Legendre stage and a_lm scatter of one job, in transform groups:
+// the matrix-core kernels write per-group partial sums into `part`
+// (FP32 for the mixed form, FP64 for the FP64 form) and the scatter adds them.
+static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
+{
+  const Job &j = s.j;
+  const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
+  if (!mixed && fp64_anal_reduce()) {
+    stage_legendre_fp64(c, s, st);
+    timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
+    return;
+  }
+  const int tmax = mixed ? ((j.spin == 0) ? kMixMaxT0 : kMixMaxTS)
+                         : ((j.spin == 0) ? kF64MaxT0 : kF64MaxTS);
+  const size_t esz = mixed ? sizeof(float) : sizeof(double);
+  const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
+  const int ng = (int)s.ml->hgmax.size();
+  const char *name = mixed ? mixed_name(j)
+                           : (j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin");
+  LegArgs &la = s.la;
+  for (int t = 0; t < j.ntrans; t += tmax) {
+    la.t0 = t;
+    la.ntl = std::min(tmax, j.ntrans - t);
+    const int npc = la.ntl * j.nalm;
+    la.part = part.get((size_t)ng * nrows * 2 * npc * esz);
+    la.part_ld = 2 * npc;
+    la.part_col0 = 0;
+    timed(c, st, name, s.triples * (s.c0 + s.c1 * la.ntl), 0.,
+          [&] { launch_manal(la, !mixed, st); });
+    s.aa.part = la.part;
+    s.aa.part_f64 = mixed ? 0 : 1;
+    s.aa.part_ld = la.part_ld;
+    s.aa.part_c0 = t * j.nalm;
+    s.aa.c0 = t * j.nalm;
+    s.aa.npc = npc;
+    timed(c, st, "alm_scatter", 0., s.alm_bytes * npc / s.ncomp_alm,
+          [&] { launch_alm_scatter(s.aa, st); });
+  }
+}
+
 static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
                        int64_t alm_base, int64_t map_base, hipStream_t st,
                        unsigned long long *opcnt)
@@ -874,54 +926,27 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt);
 
   const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
-  if (!mixed) {
-    if (j.type == SHARP_ALM2MAP) {
-      stage_gather(c, s, st);
-      stage_legendre_fp64(c, s, st);
-      stage_phase_to_rings(c, s, st);
-    } else {
-      stage_rings_to_phase(c, s, st);
-      stage_legendre_fp64(c, s, st);
-      timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
-    }
+  if (j.type != SHARP_ALM2MAP) {
+    stage_rings_to_phase(c, s, st);
+    stage_analysis(c, s, c.part, st);
     return;
   }
-
-  // mixed precision: transforms in groups of up to 16 (spin 0) or 8 (spin s)
-  // per launch, i.e. up to 32 real MFMA columns
-  const int tmax = (j.spin == 0) ? kMixMaxT0 : kMixMaxTS;
-  LegArgs &la = s.la;
-  if (j.type == SHARP_ALM2MAP) {
-    stage_gather(c, s, st);
+  stage_gather(c, s, st);
+  if (!mixed) {
+    stage_legendre_fp64(c, s, st);
+  } else {
+    // mixed precision: transforms in groups of up to 16 (spin 0) or 8 (spin
+    // s) per launch, i.e. up to 32 real MFMA columns
+    const int tmax = (j.spin == 0) ? kMixMaxT0 : kMixMaxTS;
+    LegArgs &la = s.la;
     for (int t = 0; t < j.ntrans; t += tmax) {
       la.t0 = t;
       la.ntl = std::min(tmax, j.ntrans - t);
       timed(c, st, mixed_name(j), s.triples * (s.c0 + s.c1 * la.ntl), 0.,
             [&] { launch_msynth(la, st); });
     }
-    stage_phase_to_rings(c, s, st);
-  } else {
-    stage_rings_to_phase(c, s, st);
-    const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
-    const int ng = (int)s.ml->hgmax.size();
-    for (int t = 0; t < j.ntrans; t += tmax) {
-      la.t0 = t;
-      la.ntl = std::min(tmax, j.ntrans - t);
-      const int npc = la.ntl * j.nalm;
-      la.part = static_cast<float *>(c.part.get((size_t)ng * nrows * 2 * npc * sizeof(float)));
-      la.part_ld = 2 * npc;
-      la.part_col0 = 0;
-      timed(c, st, mixed_name(j), s.triples * (s.c0 + s.c1 * la.ntl), 0.,
-            [&] { launch_manal(la, st); });
-      s.aa.part = la.part;
-      s.aa.part_ld = la.part_ld;
-      s.aa.part_c0 = t * j.nalm;
-      s.aa.c0 = t * j.nalm;
-      s.aa.npc = npc;
-      timed(c, st, "alm_scatter", 0., s.alm_bytes * npc / s.ncomp_alm,
-            [&] { launch_alm_scatter(s.aa, st); });
-    }
   }
+  stage_phase_to_rings(c, s, st);
 }
 
 // extent [lo, hi] (in real units) touched by an a_lm array
@@ -1109,8 +1134,7 @@ static void batch_run(Ctx &c, BatchPlan &b, hipStream_t st)
         stage_phase_to_rings(c, s, st);
       } else {
         stage_rings_to_phase(c, s, st);
-        stage_legendre_fp64(c, s, st);
-        timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
+        stage_analysis(c, s, b.part[i], st);
       }
     }
     return;
@@ -1355,10 +1379,10 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
       // partial sums of every transform of the job: [group][row][2*ncomp_alm]
       const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
       const size_t ng = s.ml->hgmax.size();
-      s.la.part = static_cast<float *>(
-          b->part[i].get(ng * (size_t)nrows * 2 * s.ncomp_alm * sizeof(float)));
+      s.la.part = b->part[i].get(ng * (size_t)nrows * 2 * s.ncomp_alm * sizeof(float));
       s.la.part_ld = 2 * s.ncomp_alm;
       s.aa.part = s.la.part;
+      s.aa.part_f64 = 0;
       s.aa.part_ld = s.la.part_ld;
       s.aa.part_c0 = 0;
       s.aa.c0 = 0;

@@ -28,7 +28,8 @@ struct LegArgs {
   int ntl;              // transforms in this launch
   int ngroups;          // analysis: ring-pair groups per m (partial sums)
   const int *gmax;      // analysis: largest mlim of each group [ngroups]
-  float *part;          // analysis: partial a_lm [ngroups][rows][part_ld]
+  void *part;           // analysis: partial a_lm [ngroups][rows][part_ld], float
+                        // (mixed form) or double (FP64 matrix-core form)
   int64_t part_ld;      // floats per partial row
   int part_col0;        // float column of this launch's first transform
 };
@@ -60,7 +61,11 @@ constexpr int kMixWaves0 = 8;
 constexpr int kMixWavesS = 4;
 constexpr int mix_group_slots(int spin) { return 64 * (spin == 0 ? kMixWaves0 : kMixWavesS); }
 void launch_msynth(const LegArgs &a, hipStream_t st);
-void launch_manal(const LegArgs &a, hipStream_t st);
+// f64: the FP64 form of the same kernels (FP64 matrix cores, FP64 partial
+// sums; up to 8 spin-0 / 4 spin-s transforms per launch)
+void launch_manal(const LegArgs &a, bool f64, hipStream_t st);
+constexpr int kF64MaxT0 = 8;
+constexpr int kF64MaxTS = 4;
 
 // Arguments of the ring stage (phase <-> map).
 constexpr int kMaxComp = 200;   // ntrans <= 100, two components each
@@ -127,7 +132,8 @@ struct AlmArgs {
   // components [c0, c0+npc) are sum_g part[g] over the groups g with
   // gmax[g] >= m, instead of almtmp; component c of a row is at float
   // 2*(c - part_c0) of it, rows are part_ld floats apart
-  const float *part;
+  const void *part;
+  int part_f64;         // 1: part holds doubles
   const int *gmax;
   int ngroups, c0, npc;
   int64_t part_ld;

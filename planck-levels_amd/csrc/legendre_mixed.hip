@@ -48,6 +48,28 @@ __device__ inline f4 mfma4(float a, float b, f4 c)
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 matrix products in FP32 or FP64.  A and B share one lane layout;
+// the accumulator rows differ: lane n+16q, register v holds row 4q+v in
+// FP32 and row 4v+q in FP64.
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+  typedef f4 acc;
+  static __device__ inline acc zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ inline acc run(float a, float b, acc c) { return mfma4(a, b, c); }
+  static __device__ inline int row(int q, int v) { return 4 * q + v; }
+};
+template <> struct Mfma<double> {
+  typedef d4 acc;
+  static __device__ inline acc zero() { return d4{0., 0., 0., 0.}; }
+  static __device__ inline acc run(double a, double b, acc c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int q, int v) { return 4 * v + q; }
+};
+
 // On entry v[k] of lane 16r+i belongs to lane row r; on exit v[r] of lane
 // 16k+i holds what was v[k] of lane 16r+i (a 4x4 transpose of lane rows
 // against registers).
@@ -637,14 +659,14 @@ constexpr int kTS = 20;    // LDS row stride of the [16 x 16] reduction tiles
 
 // Adds the wavefronts' tiles (in slice 0 of each wavefront) in a fixed order
 // and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm.
-template <int NT>
-__device__ inline void reduce_store(float *const *slices, int nw, float *prow, int64_t ld,
-                                    int lc, int lmax, int ncol)
+template <typename T, int NT>
+__device__ inline void reduce_store(T *const *slices, int nw, T *prow, int64_t ld, int lc,
+                                    int lmax, int ncol)
 {
   for (int e = threadIdx.x; e < NT * 256; e += blockDim.x) {
     const int tile = e >> 8, i = (e >> 4) & 15, nn = e & 15;
     const int off = (tile * 16 + i) * kTS + nn;
-    float sum = 0.f;
+    T sum = 0;
     for (int ww = 0; ww < nw; ++ww) sum += slices[ww][off];
     const int x = tile & 1, cb = tile >> 1;
     const int ll = lc + x + 2 * i, col = cb * 16 + nn;
@@ -652,8 +674,8 @@ __device__ inline void reduce_store(float *const *slices, int nw, float *prow, i
   }
 }
 
-template <int NCB>
-__device__ inline void write_tiles(float *slice, const f4 (&t)[2][NCB])
+template <typename T, int NCB>
+__device__ inline void write_tiles(T *slice, const typename Mfma<T>::acc (&t)[2][NCB])
 {
   const int lane = threadIdx.x & 63, n = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -662,13 +684,14 @@ __device__ inline void write_tiles(float *slice, const f4 (&t)[2][NCB])
     for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        slice[((cb * 2 + x) * 16 + 4 * q + v) * kTS + n] = t[x][cb][v];
+        slice[((cb * 2 + x) * 16 + Mfma<T>::row(q, v)) * kTS + n] = t[x][cb][v];
 }
 
-template <int NCB>
+template <typename T, int NCB>
 __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
-                                 float (&lds)[kMixWaves0][64 * kLS])
+                                 T (&lds)[kMixWaves0][64 * kLS])
 {
+  typedef Mfma<T> MM;
   constexpr int W = kMixWaves0;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;   // no ring pair of this group reaches m
@@ -682,11 +705,11 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
-                (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
+  T *prow = static_cast<T *>(a.part) + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
+            (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
 
   // B: p1 = N+S and p2 = N-S of ring pairs wbase+4j+kq, column n
-  float bp[2][16][NCB];
+  T bp[2][16][NCB];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int rs = wbase + 4 * j + kq;
@@ -701,8 +724,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
         nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
         if (pr) sv = ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
       }
-      bp[0][j][cb] = (float)(nv + sv);
-      bp[1][j][cb] = (float)(nv - sv);
+      bp[0][j][cb] = (T)(nv + sv);
+      bp[1][j][cb] = (T)(nv - sv);
     }
   }
 
@@ -719,8 +742,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   int mode = !wave_any(active) ? 3 : (wave_all(scale < 1) ? 0 : 1);
   double cf = corfac(scale);
   int l = m;
-  float *mine = lds[w];
-  float *slices[W];
+  T *mine = lds[w];
+  T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww];
   // recurrence coefficients of the current 8 l; the next 8 are fetched
@@ -766,8 +789,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mine[lane * kLS + 2 * j] = active ? (float)o0 : 0.f;
-        mine[lane * kLS + 2 * j + 1] = active ? (float)o1 : 0.f;
+        mine[lane * kLS + 2 * j] = active ? (T)o0 : (T)0;
+        mine[lane * kLS + 2 * j + 1] = active ? (T)o1 : (T)0;
         l += 2;
         if (q == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -785,35 +808,36 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    f4 t[2][NCB];
+    typename MM::acc t[2][NCB];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb) t[x][cb] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::zero();
     if (any) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-          const float A = mine[(4 * j + kq) * kLS + x + 2 * n];
+          const T A = mine[(4 * j + kq) * kLS + x + 2 * n];
 #pragma unroll
-          for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma4(A, bp[x][j][cb], t[x][cb]);
+          for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::run(A, bp[x][j][cb], t[x][cb]);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    write_tiles<NCB>(mine, t);
+    write_tiles<T, NCB>(mine, t);
     __syncthreads();
-    reduce_store<2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
+    reduce_store<T, 2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
     __syncthreads();
   }
 }
 
-template <int NCB>
+template <typename T, int NCB>
 __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
-                                float (&lds)[kMixWavesS][2][64 * kLS])
+                                T (&lds)[kMixWavesS][2][64 * kLS])
 {
+  typedef Mfma<T> MM;
   constexpr int W = kMixWavesS;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;
@@ -828,20 +852,20 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  float *prow = a.part + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
-                (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
+  T *prow = static_cast<T *>(a.part) + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
+            (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
 
   // rows m <= l < mhi have no lambda: zero them
   {
     const int nz = ((st.mhi < lmax + 1 ? st.mhi : lmax + 1) - m) * ncol;
     for (int e = threadIdx.x; e < nz; e += blockDim.x)
-      prow[(int64_t)(m + e / ncol) * a.part_ld + e % ncol] = 0.f;
+      prow[(int64_t)(m + e / ncol) * a.part_ld + e % ncol] = (T)0;
   }
   if (st.mhi > lmax) return;
   const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
 
   // B: P[0] = N + sg*S and P[1] = N - sg*S (columns Q.re, Q.im, U.re, U.im)
-  float bp[2][16][NCB];
+  T bp[2][16][NCB];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int rs = wbase + 4 * j + kq;
@@ -856,13 +880,13 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
         if (pr) sv = sg * ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
       }
-      bp[0][j][cb] = (float)(nv + sv);
-      bp[1][j][cb] = (float)(nv - sv);
+      bp[0][j][cb] = (T)(nv + sv);
+      bp[1][j][cb] = (T)(nv - sv);
     }
   }
   // column 4t+q of the lambda_x product takes P's column 4t+3-q with this sign
   // (a(E,B) += lw*P[x] + lx*rot(P[x^1]), libsharp/sharp_core_inc2.c:339-364)
-  const float rsgn = ((n & 3) == 0 || (n & 3) == 3) ? -1.f : 1.f;
+  const T rsgn = ((n & 3) == 0 || (n & 3) == 3) ? (T)-1 : (T)1;
 
   const double cth = a.cth[s], sth = a.sth[s];
   const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
@@ -877,8 +901,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   int mode = !wave_any(active) ? 3 : (wave_all(scalep < 1 && scalem < 1) ? 0 : 1);
   double cfp = corfac(scalep), cfm = corfac(scalem);
   int l = st.mhi;
-  float *mw = lds[w][0], *mx = lds[w][1];
-  float *slices[W];
+  T *mw = lds[w][0], *mx = lds[w][1];
+  T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
   // coefficients fx(l+1..l+4) of the current two two-steps, prefetched as in
@@ -933,10 +957,10 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mw[lane * kLS + 2 * j] = active ? (float)lw0 : 0.f;
-        mw[lane * kLS + 2 * j + 1] = active ? (float)lw1 : 0.f;
-        mx[lane * kLS + 2 * j] = active ? (float)lx0 : 0.f;
-        mx[lane * kLS + 2 * j + 1] = active ? (float)lx1 : 0.f;
+        mw[lane * kLS + 2 * j] = active ? (T)lw0 : (T)0;
+        mw[lane * kLS + 2 * j + 1] = active ? (T)lw1 : (T)0;
+        mx[lane * kLS + 2 * j] = active ? (T)lx0 : (T)0;
+        mx[lane * kLS + 2 * j + 1] = active ? (T)lx1 : (T)0;
         l += 2;
         if (h == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -951,22 +975,22 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    f4 tw[2][NCB], tx[2][NCB];
+    typename MM::acc tw[2][NCB], tx[2][NCB];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb) tw[x][cb] = tx[x][cb] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int cb = 0; cb < NCB; ++cb) tw[x][cb] = tx[x][cb] = MM::zero();
     if (any) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int off = (4 * j + kq) * kLS + x + 2 * n;
-          const float Aw = mw[off], Ax = mx[off];
+          const T Aw = mw[off], Ax = mx[off];
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb) {
-            tw[x][cb] = mfma4(Aw, bp[x][j][cb], tw[x][cb]);
-            tx[x][cb] = mfma4(Ax, bp[x ^ 1][j][cb], tx[x][cb]);
+            tw[x][cb] = MM::run(Aw, bp[x][j][cb], tw[x][cb]);
+            tx[x][cb] = MM::run(Ax, bp[x ^ 1][j][cb], tx[x][cb]);
           }
         }
 #pragma unroll
@@ -980,9 +1004,9 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    write_tiles<NCB>(mw, tw);
+    write_tiles<T, NCB>(mw, tw);
     __syncthreads();
-    reduce_store<2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
+    reduce_store<T, 2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
     __syncthreads();
   }
 }
@@ -1005,20 +1029,20 @@ __global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
   if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin_tile<NCB, DERIV>(a, mb, sb);
 }
 
-template <int NCB>
+template <typename T, int NCB>
 __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
 {
-  __shared__ float lds[kMixWaves0][64 * kLS];
+  __shared__ T lds[kMixWaves0][64 * kLS];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin0_tile<NCB>(a, m, g, lds);
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin0_tile<T, NCB>(a, m, g, lds);
 }
 
-template <int NCB>
+template <typename T, int NCB>
 __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
 {
-  __shared__ float lds[kMixWavesS][2][64 * kLS];
+  __shared__ T lds[kMixWavesS][2][64 * kLS];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin_tile<NCB>(a, m, g, lds);
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin_tile<T, NCB>(a, m, g, lds);
 }
 
 // Persistent form: workgroup b works through the items of XCD b%8 (the work
@@ -1069,7 +1093,7 @@ __global__ void __launch_bounds__(64 * kMixWaves0) kb_manal_spin0(const LegArgs 
 {
   __shared__ float lds[kMixWaves0][64 * kLS];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin0_tile<NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin0_tile<float, NCB>(job_args(jobs, it), it.major, it.minor, lds);
     __syncthreads();   // the next item reuses the LDS slices
   });
 }
@@ -1081,7 +1105,7 @@ __global__ void __launch_bounds__(64 * kMixWavesS) kb_manal_spin(const LegArgs *
 {
   __shared__ float lds[kMixWavesS][2][64 * kLS];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin_tile<NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin_tile<float, NCB>(job_args(jobs, it), it.major, it.minor, lds);
     __syncthreads();
   });
 }
@@ -1154,21 +1178,24 @@ void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items
   check_launch("batched Legendre stage");
 }
 
-void launch_manal(const LegArgs &a, hipStream_t st)
+void launch_manal(const LegArgs &a, bool f64, hipStream_t st)
 {
   const dim3 grid(xcd_grid(a.mmax + 1, a.ngroups));
   const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
-  if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
+  if (a.ntl < 1 || cols > (f64 ? 16 : 32)) SHP_FAIL("bad transform group");
   const bool two = cols > 16;
-  if (a.spin == 0) {
-    if (two) hipLaunchKernelGGL(k_manal_spin0<2>, grid, block, 0, st, a);
-    else hipLaunchKernelGGL(k_manal_spin0<1>, grid, block, 0, st, a);
+  if (f64) {
+    if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin<double, 1>), grid, block, 0, st, a);
+  } else if (a.spin == 0) {
+    if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin0<float, 1>), grid, block, 0, st, a);
   } else {
-    if (two) hipLaunchKernelGGL(k_manal_spin<2>, grid, block, 0, st, a);
-    else hipLaunchKernelGGL(k_manal_spin<1>, grid, block, 0, st, a);
+    if (two) hipLaunchKernelGGL((k_manal_spin<float, 2>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin<float, 1>), grid, block, 0, st, a);
   }
-  check_launch("mixed analysis");
+  check_launch(f64 ? "FP64 matrix-core analysis" : "mixed analysis");
 }
 
 } // namespace shp

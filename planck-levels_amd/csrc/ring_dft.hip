@@ -285,13 +285,21 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
   double2 x;
   if (a.part) {
     const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * a.part_ld;
-    const float *pp = a.part + row * a.part_ld + 2 * (comp - a.part_c0);
+    const int64_t o = row * a.part_ld + 2 * (comp - a.part_c0);
     double re = 0., im = 0.;
     for (int g = 0; g < a.ngroups; ++g) {
       if (a.gmax[g] < m) continue;
-      const float2 v = *reinterpret_cast<const float2 *>(pp + g * gstride);
-      re += v.x;
-      im += v.y;
+      if (a.part_f64) {
+        const double2 v = *reinterpret_cast<const double2 *>(
+            static_cast<const double *>(a.part) + o + g * gstride);
+        re += v.x;
+        im += v.y;
+      } else {
+        const float2 v = *reinterpret_cast<const float2 *>(
+            static_cast<const float *>(a.part) + o + g * gstride);
+        re += v.x;
+        im += v.y;
+      }
     }
     x = make_double2(re, im);
   } else {
