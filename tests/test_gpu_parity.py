@@ -161,3 +161,14 @@ def test_cfg1_spin0_vs_reference(product, ref, gpu):
     """HEALPix nside 256, lmax 512 spin 0 in both directions."""
     for jt in (O.ALM2MAP, O.MAP2ALM):
         compare(product, ref, healpix(256), tri(512), jt, 0, 1, O.DP)
+
+
+@pytest.mark.parametrize("spin", [0, 2, 3])
+def test_fp64_analysis_reduction_kernels(product, oracle, gpu, spin, monkeypatch):
+    """SHARP_HIP_FP64_ANALYSIS=reduce selects the FP64 analysis kernels that sum
+    over ring pairs with lane reductions instead of matrix cores; both
+    directions of analysis (map2alm, Y^T) match the oracle to 1e-12."""
+    monkeypatch.setenv("SHARP_HIP_FP64_ANALYSIS", "reduce")
+    for jt in (O.MAP2ALM, O.Yt):
+        compare(product, oracle, healpix(16), tri(32), jt, spin, 3, O.DP)
+        compare(product, oracle, iso("gauss", 25, 49), tri(24), jt, spin, 5, O.DP)
