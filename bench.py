@@ -254,6 +254,75 @@ def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
     return res
 
 
+def smoothing_workload(S, torch, dev, args):
+    """smoothing_cxx's transform work on the TQU batch (Nside 2048, lmax 4096,
+    8 realisations): map2alm_iter with 3 Jacobi iterations, then alm2map of
+    the a_lm times a 5' Gaussian beam (the beam applied while the a_lm are
+    gathered), for T (spin 0) and Q/U (spin 2), all on the GPU.  One step
+    smooths the 8 TQU maps: 4 analyses + 4 syntheses per spin.  Both forms
+    are timed; the error of the mixed form is stated against the FP64 form."""
+    nside = args.nside
+    lmax = args.lmax if args.lmax is not None else 2 * nside
+    batch, niter = args.batch, 3
+    fwhm = math.radians(5.0 / 60.0)
+    npix = 12 * nside * nside
+    geom = S.Geometry.healpix(nside)
+    ainfo = S.AlmInfo.triangular(lmax)
+    log("generating maps: nside %d lmax %d batch %d" % (nside, lmax, batch))
+    T, E, B = make_alms(torch, lmax, batch, args.seed, dev)
+    EB = [x for pair in zip(E, B) for x in pair]
+    m_t = [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(batch)]
+    m_qu = [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(2 * batch)]
+    S.execute_device(S.ALM2MAP, 0, T, m_t, geom, ainfo, batch, S.DP)
+    S.execute_device(S.ALM2MAP, 2, EB, m_qu, geom, ainfo, batch, S.DP)
+    torch.cuda.synchronize()
+    del T, E, B, EB
+    res = {}
+    for form in ("mixed", "fp64"):
+        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+        wt = [x.clone() for x in m_t]
+        wqu = [x.clone() for x in m_qu]
+        at = [torch.zeros(S.nalm_triangular(lmax), dtype=torch.complex128, device=dev)
+              for _ in range(batch)]
+        aeb = [torch.zeros_like(at[0]) for _ in range(2 * batch)]
+
+        def step():
+            # smoothing_cxx maps in place: restore the inputs first
+            for d, x in zip(wt + wqu, m_t + m_qu):
+                d.copy_(x)
+            S.smoothing_device(wt, geom, ainfo, 0, fwhm, niter, at, flags)
+            S.smoothing_device(wqu, geom, ainfo, 2, fwhm, niter, aeb, flags)
+
+        log("%s: warmup x%d, timed x%d" % (form, args.warmup, args.steps))
+        dt, stats = run_form(S, torch, step, args.warmup, args.steps)
+        ms = 1e3 * dt / args.steps
+        val = batch * args.steps / dt
+        show(form, stats, ms, val)
+        res[form] = {"value": val, "ms_per_step": ms, "maps": wt + wqu,
+                     "roofline": roofline(stats, form == "mixed", batch)}
+
+    def rel(x, y):
+        return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+    err = max(rel(x, y) for x, y in zip(res["mixed"]["maps"], res["fp64"]["maps"]))
+    out = {
+        "metric": "smoothing_cxx TQU maps/s, Nside %d lmax %d, 3 iterations, 5' beam" % (nside, lmax),
+        "value": res["mixed"]["value"], "unit": "TQU maps smoothed/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mfma contraction, f64 recurrence/FFT",
+        "data": "synthetic: TQU maps synthesised from the seeded a_lm of the headline workload",
+        "config": {"workload": "smoothing_cxx core (map2alm_iter, smoothWithGauss, alm2map) "
+                               "of 8 TQU maps", "nside": nside, "lmax": lmax,
+                   "batch": batch, "iterations": niter, "fwhm_arcmin": 5.0},
+        "roofline": res["mixed"]["roofline"],
+        "fp64_form": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                      "roofline": res["fp64"]["roofline"]},
+        "mixed_vs_fp64_max_rel_l2": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def batch_workload(S, torch, dev, args):
     """The uneven batch: 24 spin-0 maps each at Nside 128, 256, 512, 1024
     (lmax = 2 Nside) and 8 Gauss-Legendre grids each at lmax 255 and 767
@@ -354,9 +423,11 @@ def main():
     ap.add_argument("--host-path", action="store_true",
                     help="also time sharp_execute on host arrays (H2D/D2H copies "
                          "included; reported beside value, never as value)")
-    ap.add_argument("--workload", default="tqu", choices=["tqu", "batch"],
+    ap.add_argument("--workload", default="tqu", choices=["tqu", "batch", "smoothing"],
                     help="tqu: the headline workload; batch: the uneven batch "
-                         "(96 HEALPix maps of four sizes + 16 Gauss-Legendre grids)")
+                         "(96 HEALPix maps of four sizes + 16 Gauss-Legendre grids); "
+                         "smoothing: smoothing_cxx of the TQU batch (3 Jacobi "
+                         "iterations, 5' beam)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU form of this path "
@@ -371,6 +442,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload == "batch":
         batch_workload(S, torch, dev, args)
+        return
+    if args.workload == "smoothing":
+        smoothing_workload(S, torch, dev, args)
         return
     nside = args.nside
     lmax = args.lmax if args.lmax is not None else 2 * nside

@@ -45,6 +45,26 @@ void sharp_hip_map2alm_adjoint_device (int spin, void *alm, void *map,
   const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
   int ntrans, int flags, void *stream);
 
+/* sharp_hip_execute_device with a per-l factor on the a_lm side: synthesis
+   computes map = Y (f_l a_lm) without modifying the caller's a_lm, analysis
+   a_lm (+)= f_l (Y^T W map).  Replaces Alm::ScaleL before alm2map or after
+   map2alm, e.g. smoothWithGauss (Healpix_cxx/alm_powspec_tools.cc:250-283)
+   followed by alm2map in smoothing_cxx (Healpix_cxx/smoothing_cxx_module.cc:
+   79-81), with the factor applied while the a_lm are gathered or scattered.
+   lfac: host array of lmax+1 doubles, read before the call returns. */
+void sharp_hip_execute_scaled_device (sharp_jobtype type, int spin, void *alm,
+  void *map, const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+  int ntrans, int flags, const double *lfac, void *stream);
+
+/* Jacobi-iterated analysis, map2alm_iter / map2alm_pol_iter
+   (Healpix_cxx/alm_healpix_tools.cc:73-85, 239-265) on device arrays:
+   a = Y^T W m, then num_iter times a += Y^T W (m - Y a).  One spin per call
+   (T and E/B of a polarised map are independent).  The residual maps live in
+   a library work buffer; SHARP_ADD is rejected. */
+void sharp_hip_map2alm_iter_device (int spin, void *alm, void *map,
+  const sharp_geom_info *geom_info, const sharp_alm_info *alm_info,
+  int ntrans, int flags, int num_iter, void *stream);
+
 /* Batches of transforms of uneven size (several geometries, lmax, spins and
    transform counts), no counterpart in the reference (whose callers issue one
    sharp_execute per geometry).  `alm`/`map` are host arrays of device

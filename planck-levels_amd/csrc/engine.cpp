@@ -213,7 +213,12 @@ struct Ctx {
   std::vector<std::unique_ptr<YlmPlan>> ylms;
   std::vector<std::unique_ptr<NormPlan>> norms;
   std::vector<std::unique_ptr<AlmPlan>> alms;
-  DevBuf phase, almtmp, part, stage_alm, stage_map;
+  DevBuf phase, almtmp, part, stage_alm, stage_map, resid, lfac;
+  // pinned staging of the per-l factors; lfac_ev marks the end of the last
+  // copy out of it
+  double *lfac_host = nullptr;
+  size_t lfac_cap = 0;
+  hipEvent_t lfac_ev = nullptr;
 };
 
 static std::mutex g_mutex;
@@ -528,10 +533,8 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
 }
 
 // norm_l (libsharp/sharp_ylmgen_c.c:200-232)
-static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
+static std::vector<double> norm_values(int lmax, int spin, int deriv)
 {
-  for (auto &p : c.norms)
-    if (p->lmax == lmax && p->spin == spin && p->deriv == deriv) return p->d;
   std::vector<double> n(lmax + 1);
   if (deriv) {
     for (int l = 0; l <= lmax; ++l)
@@ -544,6 +547,14 @@ static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
     for (int l = 0; l <= lmax; ++l)
       n[l] = (l < spin) ? 0. : spinsign * 0.5 * std::sqrt((2 * l + 1) / (4 * kPi));
   }
+  return n;
+}
+
+static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
+{
+  for (auto &p : c.norms)
+    if (p->lmax == lmax && p->spin == spin && p->deriv == deriv) return p->d;
+  const std::vector<double> n = norm_values(lmax, spin, deriv);
   auto np = std::make_unique<NormPlan>();
   np->lmax = lmax;
   np->spin = spin;
@@ -581,6 +592,7 @@ struct Job {
   int type, spin, nmaps, nalm, ntrans, flags, lmax, mmax, deriv;
   const sharp_geom_info *g;
   const sharp_alm_info *a;
+  const double *lfac = nullptr;   // optional f_l applied on the a_lm side
 };
 
 static Job make_job(int type, int spin, const sharp_geom_info *g,
@@ -687,9 +699,29 @@ static size_t almtmp_elems(const Job &j)
   return ((size_t)tri_ofs(j.mmax + 1, j.lmax) + kAlmSlackRows) * j.ntrans * j.nalm;
 }
 
+// norm_l * f_l on the device for a job with per-l factors, copied on `st`
+// through pinned memory (reused once the previous copy out of it has ended)
+static double *upload_lfac(Ctx &c, const Job &j, hipStream_t st)
+{
+  std::vector<double> n = norm_values(j.lmax, j.spin, j.deriv);
+  const size_t bytes = n.size() * sizeof(double);
+  if (c.lfac_ev) HIP_CHECK(hipEventSynchronize(c.lfac_ev));
+  else HIP_CHECK(hipEventCreateWithFlags(&c.lfac_ev, hipEventDisableTiming));
+  if (bytes > c.lfac_cap) {
+    if (c.lfac_host) HIP_CHECK(hipHostFree(c.lfac_host));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c.lfac_host), bytes));
+    c.lfac_cap = bytes;
+  }
+  for (size_t l = 0; l < n.size(); ++l) c.lfac_host[l] = n[l] * j.lfac[l];
+  double *d = static_cast<double *>(c.lfac.get(bytes));
+  HIP_CHECK(hipMemcpyAsync(d, c.lfac_host, bytes, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipEventRecord(c.lfac_ev, st));
+  return d;
+}
+
 static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
                           int64_t alm_base, int64_t map_base, double2 *phase,
-                          double2 *almtmp, unsigned long long *opcnt)
+                          double2 *almtmp, unsigned long long *opcnt, hipStream_t st)
 {
   JobState s;
   s.j = j;
@@ -717,7 +749,7 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   aa.stride = j.a->stride;
   aa.alm_base = alm_base;
   aa.mvstart_by_m = d_mvs;
-  aa.norm_l = d_norm;
+  aa.norm_l = j.lfac ? upload_lfac(c, j, st) : d_norm;
   aa.almtmp = almtmp;
   for (int i = 0; i < ncomp_alm; ++i) aa.alms.p[i] = dalm[i];
   aa.part = nullptr;
@@ -918,7 +950,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   GeomPlan &gp = get_geom(c, j.g);
   double2 *phase = static_cast<double2 *>(c.phase.get(phase_elems(j, gp) * sizeof(double2)));
   double2 *almtmp = static_cast<double2 *>(c.almtmp.get(almtmp_elems(j) * sizeof(double2)));
-  JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt);
+  JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
 
   const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
   if (j.type != SHARP_ALM2MAP) {
@@ -1337,6 +1369,64 @@ void sharp_hip_map2alm_adjoint_device(int spin, void *alm, void *map,
                            flags, stream, nullptr);
 }
 
+void sharp_hip_execute_scaled_device(sharp_jobtype type, int spin, void *alm, void *map,
+                                     const sharp_geom_info *geom_info,
+                                     const sharp_alm_info *alm_info, int ntrans, int flags,
+                                     const double *lfac, void *stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Job j = make_job(type, spin, geom_info, alm_info, ntrans, flags);
+  if (!lfac) SHP_FAIL("per-l factors missing");
+  j.lfac = lfac;
+  Ctx &c = context();
+  run_device(c, j, static_cast<void *const *>(alm), static_cast<void *const *>(map), 0, 0,
+             static_cast<hipStream_t>(stream), nullptr);
+}
+
+void sharp_hip_map2alm_iter_device(int spin, void *alm, void *map,
+                                   const sharp_geom_info *geom_info,
+                                   const sharp_alm_info *alm_info, int ntrans, int flags,
+                                   int num_iter, void *stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (num_iter < 0) SHP_FAIL("bad number of iterations");
+  if (flags & SHARP_ADD) SHP_FAIL("SHARP_ADD is not supported by the iterated analysis");
+  const Job ja = make_job(SHARP_MAP2ALM, spin, geom_info, alm_info, ntrans, flags);
+  Ctx &c = context();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  void *const *dalm = static_cast<void *const *>(alm);
+  void *const *dmap = static_cast<void *const *>(map);
+  run_device(c, ja, dalm, dmap, 0, 0, st, nullptr);
+  if (num_iter == 0) return;
+  const Job js = make_job(SHARP_ALM2MAP, spin, geom_info, alm_info, ntrans, flags);
+  const Job jadd = make_job(SHARP_MAP2ALM, spin, geom_info, alm_info, ntrans, flags | SHARP_ADD);
+  // residual maps over the pixel extent of the geometry (reals; complex
+  // pixels under SHARP_NO_FFT)
+  int64_t mlo, mhi;
+  map_extent(geom_info, mlo, mhi);
+  const bool nofft = flags & SHARP_NO_FFT;
+  const int64_t rlo = nofft ? 2 * mlo : mlo, rhi = nofft ? 2 * mhi + 1 : mhi;
+  const int64_t n = rhi - rlo + 1;
+  const int ncomp = ja.ntrans * ja.nmaps;
+  const size_t esz = (flags & SHARP_DP) ? sizeof(double) : sizeof(float);
+  char *rb = static_cast<char *>(c.resid.get((size_t)n * esz * ncomp));
+  std::vector<void *> rmap(ncomp);
+  PtrTab src{}, dst{};
+  for (int i = 0; i < ncomp; ++i) {
+    rmap[i] = rb + (size_t)i * n * esz;
+    src.p[i] = static_cast<char *>(dmap[i]) + rlo * (int64_t)esz;
+    dst.p[i] = rmap[i];
+  }
+  for (int it = 0; it < num_iter; ++it) {
+    // Healpix_cxx/alm_healpix_tools.cc:77-84: map2 = map - alm2map(alm);
+    // alm += map2alm(map2)
+    run_device(c, js, dalm, rmap.data(), 0, mlo, st, nullptr);
+    timed(c, st, "map_residual", 0., 3. * (double)n * esz * ncomp,
+          [&] { launch_map_residual(src, dst, ncomp, n, (flags & SHARP_DP) != 0, st); });
+    run_device(c, jadd, dalm, rmap.data(), 0, mlo, st, nullptr);
+  }
+}
+
 sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, int flags,
                                         int use_graph)
 {
@@ -1368,7 +1458,8 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
     GeomPlan &gp = get_geom(c, j.g);
     double2 *ph = static_cast<double2 *>(b->phase[i].get(phase_elems(j, gp) * sizeof(double2)));
     double2 *at = static_cast<double2 *>(b->almtmp[i].get(almtmp_elems(j) * sizeof(double2)));
-    b->st.push_back(setup_job(c, j, b->almp[i].data(), b->mapp[i].data(), 0, 0, ph, at, nullptr));
+    b->st.push_back(setup_job(c, j, b->almp[i].data(), b->mapp[i].data(), 0, 0, ph, at, nullptr,
+                              c.stream));
     JobState &s = b->st.back();
     if (b->mixed && !b->synth) {
       // partial sums of every transform of the job: [group][row][2*ncomp_alm]
@@ -1484,6 +1575,8 @@ void sharp_hip_release(void)
   c.part.release();
   c.stage_alm.release();
   c.stage_map.release();
+  c.resid.release();
+  c.lfac.release();
 }
 
 void sharp_hip_profile(int enable)

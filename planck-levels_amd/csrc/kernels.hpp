@@ -139,6 +139,9 @@ struct AlmArgs {
   int64_t part_ld;
   int part_c0;
 };
+// dst[c][k] = src[c][k] - dst[c][k] for k < n, c < ncomp (float or double)
+void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_t n,
+                         bool dp, hipStream_t st);
 void launch_alm_gather(const AlmArgs &a, hipStream_t st);
 void launch_alm_scatter(const AlmArgs &a, hipStream_t st);
 

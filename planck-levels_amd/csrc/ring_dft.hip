@@ -17,6 +17,7 @@
 // applying norm_l and the REAL_HARMONICS m=0 factor
 // (libsharp/sharp.c:481-606).
 
+#include <algorithm>
 #include <hip/hip_runtime.h>
 #include "kernels.hpp"
 #include "sharp_lowlevel.h"
@@ -330,6 +331,26 @@ void launch_alm_gather(const AlmArgs &a, hipStream_t s)
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_gather<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_gather<float>, grid, dim3(256), 0, s, a);
   check_launch("alm_gather");
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_map_residual(PtrTab src, PtrTab dst, int64_t n)
+{
+  const T *x = static_cast<const T *>(src.p[blockIdx.y]);
+  T *y = static_cast<T *>(dst.p[blockIdx.y]);
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * 256)
+    y[k] = x[k] - y[k];
+}
+
+void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_t n,
+                         bool dp, hipStream_t s)
+{
+  if (n <= 0 || ncomp <= 0) return;
+  const dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 4096), ncomp);
+  if (dp) hipLaunchKernelGGL(k_map_residual<double>, grid, dim3(256), 0, s, src, dst, n);
+  else hipLaunchKernelGGL(k_map_residual<float>, grid, dim3(256), 0, s, src, dst, n);
+  check_launch("map_residual");
 }
 
 void launch_alm_scatter(const AlmArgs &a, hipStream_t s)

@@ -17,6 +17,7 @@ If the library is missing this module raises at import: there is no CPU
 fallback.
 """
 import ctypes as ct
+import math
 import os
 
 import numpy as np
@@ -98,6 +99,12 @@ def _load():
               "sharp_hip_map2alm_adjoint_device"):
         getattr(lib, f).argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, G, A,
                                     ct.c_int, ct.c_int, ct.c_void_p]
+    lib.sharp_hip_execute_scaled_device.argtypes = [
+        ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int,
+        ct.POINTER(ct.c_double), ct.c_void_p]
+    lib.sharp_hip_map2alm_iter_device.argtypes = [
+        ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int, ct.c_int,
+        ct.c_void_p]
     lib.sharp_hip_batch_create.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_int]
     lib.sharp_hip_batch_create.restype = ct.c_void_p
     lib.sharp_hip_batch_execute.argtypes = [ct.c_void_p, ct.c_void_p]
@@ -277,6 +284,70 @@ def map2alm_adjoint_device(spin, alms, maps, geom, ainfo, ntrans, flags, stream=
     quadrature weights, libsharp's SHARP_WY)."""
     _dev_call(LIB.sharp_hip_map2alm_adjoint_device, spin, alms, maps, geom, ainfo,
               ntrans, flags, stream)
+
+
+def execute_scaled_device(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags, lfac,
+                          stream=None):
+    """execute_device with the per-l factor lfac (lmax+1 values) on the a_lm
+    side: synthesis maps = Y (lfac_l a_lm) (the caller's a_lm unchanged),
+    analysis a_lm (+)= lfac_l Y^T W maps."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    f = np.ascontiguousarray(lfac, dtype=np.float64)
+    if f.size < ainfo.lmax + 1:
+        raise ValueError("lfac needs lmax+1 values")
+    ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+    LIB.sharp_hip_execute_scaled_device(
+        jobtype, spin, ap, mp, geom.ptr, ainfo.ptr, ntrans, flags,
+        f.ctypes.data_as(ct.POINTER(ct.c_double)), ct.c_void_p(stream.cuda_stream))
+
+
+def map2alm_iter_device(spin, alms, maps, geom, ainfo, ntrans, flags, num_iter,
+                        stream=None):
+    """Jacobi-iterated analysis on device tensors (Healpix_cxx map2alm_iter):
+    alms = Y^T W maps, then num_iter times alms += Y^T W (maps - Y alms)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+    LIB.sharp_hip_map2alm_iter_device(spin, ap, mp, geom.ptr, ainfo.ptr, ntrans, flags,
+                                      num_iter, ct.c_void_p(stream.cuda_stream))
+
+
+FWHM2SIGMA = 1.0 / math.sqrt(8.0 * math.log(2.0))
+
+
+def gauss_beam(lmax, fwhm, pol=False):
+    """Per-l factors of Healpix's smoothWithGauss (Healpix_cxx/
+    alm_powspec_tools.cc:250-283): exp(-l(l+1) sigma^2 / 2) with sigma =
+    |fwhm| * fwhm2sigma (radians), the inverse for fwhm < 0, and the extra
+    exp(2 sigma^2) (or its inverse) of the polarisation a_lm when pol."""
+    fct = 1 if fwhm >= 0 else -1
+    sigma = fwhm * FWHM2SIGMA
+    l = np.arange(lmax + 1, dtype=np.float64)
+    gb = np.exp(-0.5 * fct * l * (l + 1) * sigma * sigma)
+    if pol:
+        gb = gb * math.exp(2 * fct * sigma * sigma)
+    return gb
+
+
+def smoothing_device(maps, geom, ainfo, spin, fwhm, num_iter, alms, flags=None,
+                     stream=None):
+    """smoothing_cxx on device tensors (Healpix_cxx/smoothing_cxx_module.cc:
+    79-81, 108-111) for one spin: alms = map2alm_iter(maps), then
+    maps = alm2map(beam_l * alms) with the Gaussian beam of `fwhm` (radians)
+    applied while the a_lm are gathered.  `alms` receive the unsmoothed a_lm.
+    ntrans = len(maps) / (1 for spin 0, 2 otherwise)."""
+    if flags is None:
+        flags = DP
+    nc = 1 if spin == 0 else 2
+    ntrans = len(maps) // nc
+    map2alm_iter_device(spin, alms, maps, geom, ainfo, ntrans, flags, num_iter, stream)
+    execute_scaled_device(ALM2MAP, spin, alms, maps, geom, ainfo, ntrans, flags,
+                          gauss_beam(ainfo.lmax, fwhm, pol=spin != 0), stream)
 
 
 class _JobC(ct.Structure):
