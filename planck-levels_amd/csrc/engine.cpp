@@ -108,9 +108,9 @@ struct GeomPlan {
   std::vector<Mlim> mlims;
   // ring transforms: FFT plans grouped by LDS class, the rings too long for
   // any class (direct summation), and the list of all rings (SHARP_NO_FFT)
-  RingFftPlan *fft[2 * kNumFftClasses] = {};   // [2*class + bluestein]
-  std::vector<RingFftPlan> hfft[2 * kNumFftClasses];
-  int nfft[2 * kNumFftClasses] = {};
+  RingFftPlan *fft[kFftKinds * kNumFftClasses] = {};   // [kFftKinds*class + kind]
+  std::vector<RingFftPlan> hfft[kFftKinds * kNumFftClasses];
+  int nfft[kFftKinds * kNumFftClasses] = {};
   double2 *btab = nullptr;   // Bluestein chirp spectra
   int *dft_list = nullptr, ndft = 0;
   int *all_list = nullptr, nall = 0;
@@ -348,7 +348,7 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
   }
   gp->nphmax = nphmax;
   // ring FFT plans by LDS class; rings beyond every class use direct sums
-  std::vector<RingFftPlan> cls[2 * kNumFftClasses];
+  std::vector<RingFftPlan> cls[kFftKinds * kNumFftClasses];
   std::vector<int> dft, all;
   for (int rid = 0; rid < 2 * P; ++rid) {
     const int n = ring[rid].nph;
@@ -362,7 +362,13 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     gp->npix_all += n;
     if (cl < 0) dft.push_back(rid);
     else {
-      cls[2 * cl + pl.bluestein].push_back(pl);
+      // radix set: 2 only 8, 1 only 8, 4, 2, 0 any
+      int rs = 2;
+      for (int q = 0; q < pl.nrad; ++q) {
+        if (pl.rad[q] == 4 || pl.rad[q] == 2) rs = std::min(rs, 1);
+        else if (pl.rad[q] != 8) rs = 0;
+      }
+      cls[kFftKinds * cl + pl.bluestein + 2 * rs].push_back(pl);
       gp->npix_fft += n;
     }
   }
@@ -370,8 +376,10 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
   std::map<std::pair<int, int>, int64_t> boffs;
   int64_t btot = 0;
   FftClasses uniq{};
-  std::vector<RingFftPlan> ucls[2 * kNumFftClasses];
-  for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
+  // (the table kernels take every Bluestein plan of a class from slot
+  // kFftKinds*class + 1, whatever its radices)
+  std::vector<RingFftPlan> ucls[kFftKinds * kNumFftClasses];
+  for (int c2 = 0; c2 < kFftKinds * kNumFftClasses; ++c2) {
     for (auto &pl : cls[c2]) {
       pl.boff = -1;
       if (!pl.bluestein) continue;
@@ -379,10 +387,10 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
       auto it = boffs.find(key);
       if (it == boffs.end()) {
         it = boffs.emplace(key, btot).first;
-        btot += pl.M;
+        btot += pl.M + pl.N;   // kernel spectrum, then the chirp
         RingFftPlan u = pl;
         u.boff = it->second;
-        ucls[c2].push_back(u);
+        ucls[(c2 / kFftKinds) * kFftKinds + 1].push_back(u);
       }
       pl.boff = it->second;
     }
@@ -396,8 +404,8 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
   }
   if (btot > 0) {
     HIP_CHECK(hipMalloc(&gp->btab, (size_t)btot * sizeof(double2)));
-    std::vector<RingFftPlan *> dev(2 * kNumFftClasses, nullptr);
-    for (int c2 = 0; c2 < 2 * kNumFftClasses; ++c2) {
+    std::vector<RingFftPlan *> dev(kFftKinds * kNumFftClasses, nullptr);
+    for (int c2 = 0; c2 < kFftKinds * kNumFftClasses; ++c2) {
       uniq.count[c2] = (int)ucls[c2].size();
       if (!ucls[c2].empty()) dev[c2] = upload(ucls[c2]);
       uniq.plans[c2] = dev[c2];
@@ -803,7 +811,7 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
 
   int64_t nring_fft = 0;
   s.fc.btab = gp.btab;
-  for (int cl = 0; cl < 2 * kNumFftClasses; ++cl) {
+  for (int cl = 0; cl < kFftKinds * kNumFftClasses; ++cl) {
     s.fc.plans[cl] = gp.fft[cl];
     s.fc.hplans[cl] = gp.hfft[cl].data();
     s.fc.count[cl] = gp.nfft[cl];

@@ -102,15 +102,19 @@ struct RingFftPlan {
   int M;           // Bluestein convolution length (7-smooth, >= 2N-1)
   int nrad;
   int8_t rad[16];
-  int64_t boff;    // Bluestein: offset of the chirp spectrum in the table
+  int64_t boff;    // Bluestein: offset of the plan's block in the table
+                   // (M kernel-spectrum entries, then N chirp entries)
 };
 constexpr int kNumFftClasses = 5;   // LDS buffers of 512 << c complex doubles
-// plans[2*class + bluestein], sorted by decreasing ring length; hplans are
-// host copies of the same arrays
+// kinds of plan in a class: bluestein + 2 * radix set (0: 8, 4, 2, 3, 5, 7;
+// 1: 8, 4, 2 only; 2: 8 only)
+constexpr int kFftKinds = 6;
+// plans[kFftKinds*class + kind], sorted by decreasing ring length; hplans
+// are host copies of the same arrays
 struct FftClasses {
-  const RingFftPlan *plans[2 * kNumFftClasses];
-  const RingFftPlan *hplans[2 * kNumFftClasses];
-  int count[2 * kNumFftClasses];
+  const RingFftPlan *plans[kFftKinds * kNumFftClasses];
+  const RingFftPlan *hplans[kFftKinds * kNumFftClasses];
+  int count[kFftKinds * kNumFftClasses];
   const double2 *btab;   // Bluestein chirp spectra
 };
 // Fills btab at the offsets of the given plans (one plan per distinct (N, M)).

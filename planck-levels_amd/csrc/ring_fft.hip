@@ -19,8 +19,11 @@
 // 5 and 7, in place in one LDS buffer (every pass reads all its inputs into
 // registers, barriers, then writes).  A length with another prime factor is
 // done by Bluestein's algorithm: a circular convolution of length
-// M = 2^k >= 2N-1 with the chirp e^{-i pi j^2 / N}; the chirp's FFT is formed
-// once per ring and kept in registers for all components.
+// M >= 2N-1 (7-smooth) with the chirp e^{-i pi j^2 / N}; the chirp and the
+// FFT of the convolution kernel are precomputed once per (N, M) into a table
+// in HBM.  Twiddles of a pass come from one sincos per butterfly; the
+// azimuthal rotations e^{+-i m phi0} from one sincos per group of bins,
+// advanced by complex multiplication.
 //
 // Rings are grouped by the LDS buffer they need (512 ... 8192 complex
 // doubles); each group is one launch of a kernel instantiated for that size
@@ -197,13 +200,17 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
     }
   }
   __syncthreads();
+  // twiddle angle per unit of k, in units of pi; k = j mod Ns by a mask when
+  // Ns is a power of two (every pass of radix 8, 4, 2)
+  const double tscale = (double)sign * 2.0 / (double)(Ns * R);
+  const bool pow2 = (Ns & (Ns - 1)) == 0;
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int j = tid + i * NT;
     if (j < nb) {
-      const int k = j % Ns;
+      const int k = pow2 ? (j & (Ns - 1)) : (j % Ns);
       if (Ns > 1) {
-        const double2 w = expipi(sign * 2.0 * k / (double)(Ns * R));
+        const double2 w = expipi((double)k * tscale);
         double2 wr = w;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -212,7 +219,7 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
         }
       }
       Dft<R>::run(v[i], sign);
-      const int base = (j / Ns) * Ns * R + k;
+      const int base = (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[i][r];
     }
@@ -225,19 +232,28 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
 // the passes are laid out in that order as straight-line code (a switch
 // inside one loop makes the register allocator keep every pass's values
 // live at once).
-template <int NT, int E>
+// P2 (radix set of the plan): 0 any of 8, 4, 2, 3, 5, 7; 1 only 8, 4, 2 (a
+// power-of-two length); 2 only 8.  The passes a plan cannot have are left
+// out of the kernel, which keeps its register count (and so its occupancy)
+// at what the passes it runs need.
+template <int NT, int E, int P2>
 __device__ void lds_fft(double2 *buf, int N, const int8_t *rad, int nrad, int sign)
 {
   int Ns = 1, s = 0;
   for (; s < nrad && rad[s] == 8; ++s, Ns *= 8) stockham_pass<NT, E, 8>(buf, N, Ns, sign);
-  if (s < nrad && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }
-  if (s < nrad && rad[s] == 2) { stockham_pass<NT, E, 2>(buf, N, Ns, sign); Ns *= 2; ++s; }
-  for (; s < nrad && rad[s] == 3; ++s, Ns *= 3) stockham_pass<NT, E, 3>(buf, N, Ns, sign);
-  for (; s < nrad && rad[s] == 5; ++s, Ns *= 5) stockham_pass<NT, E, 5>(buf, N, Ns, sign);
-  for (; s < nrad && rad[s] == 7; ++s, Ns *= 7) stockham_pass<NT, E, 7>(buf, N, Ns, sign);
+  if constexpr (P2 < 2) {
+    if (s < nrad && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }
+    if (s < nrad && rad[s] == 2) { stockham_pass<NT, E, 2>(buf, N, Ns, sign); Ns *= 2; ++s; }
+  }
+  if constexpr (P2 == 0) {
+    for (; s < nrad && rad[s] == 3; ++s, Ns *= 3) stockham_pass<NT, E, 3>(buf, N, Ns, sign);
+    for (; s < nrad && rad[s] == 5; ++s, Ns *= 5) stockham_pass<NT, E, 5>(buf, N, Ns, sign);
+    for (; s < nrad && rad[s] == 7; ++s, Ns *= 7) stockham_pass<NT, E, 7>(buf, N, Ns, sign);
+  }
 }
 
-// Bluestein chirp h_j = e^{sign i pi j^2 / N} with the exact angle
+// Bluestein chirp h_j = e^{sign i pi j^2 / N} with the exact angle (table
+// construction only; the transforms read it from the table)
 __device__ inline double2 chirp(int64_t j, int N, int sign)
 {
   const int64_t q = (j * j) % (2 * (int64_t)N);
@@ -257,23 +273,26 @@ __device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
     buf[j] = (d >= 0) ? chirp(d, N, -sign) : make_double2(0., 0.);
   }
   __syncthreads();
-  lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
+  lds_fft<NT, E, 0>(buf, M, P.rad, P.nrad, -1);
 }
 
 // length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein;
-// Bt is the spectrum of the Bluestein kernel (see bluestein_kernel), read
-// from the table in global memory
-template <int NT, int E, bool BLUE>
+// Bt is the plan's block of the Bluestein table: the spectrum of the
+// convolution kernel (M entries, see bluestein_kernel), then the chirp
+// e^{-i pi j^2/N} (N entries)
+template <int NT, int E, bool BLUE, int P2>
 __device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
   if constexpr (!BLUE) {
-    lds_fft<NT, E>(buf, P.N, P.rad, P.nrad, sign);
+    lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
   } else {
   const int N = P.N, M = P.M;
+  const double2 *Ct = Bt + M;
+  auto ch = [&](int j) { const double2 c = Ct[j]; return sign > 0 ? conj(c) : c; };
   for (int j = threadIdx.x; j < M; j += NT)
-    buf[j] = (j < N) ? cmul(buf[j], chirp(j, N, sign)) : make_double2(0., 0.);
+    buf[j] = (j < N) ? cmul(buf[j], ch(j)) : make_double2(0., 0.);
   __syncthreads();
-  lds_fft<NT, E>(buf, M, P.rad, P.nrad, -1);
+  lds_fft<NT, E, P2>(buf, M, P.rad, P.nrad, -1);
   // the kernel of sign s is the conjugate of the kernel of sign -s with the
   // input conjugated: FFT(conj(x))_k = conj(FFT(x)_{-k}); the table holds
   // the spectrum for the analysis sign -1
@@ -283,10 +302,10 @@ __device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const dou
     buf[j] = cmul(buf[j], b);
   }
   __syncthreads();
-  lds_fft<NT, E>(buf, M, P.rad, P.nrad, +1);
+  lds_fft<NT, E, P2>(buf, M, P.rad, P.nrad, +1);
   const double inv = 1.0 / M;
   for (int j = threadIdx.x; j < N; j += NT)
-    buf[j] = cscale(cmul(buf[j], chirp(j, N, sign)), inv);
+    buf[j] = cscale(cmul(buf[j], ch(j)), inv);
   __syncthreads();
   }
 }
@@ -331,7 +350,7 @@ __device__ inline void batched(int lo, int hi, L &&load, C &&consume)
 // ===========================================================================
 // synthesis: phase -> ring
 // ===========================================================================
-template <int S, bool BLUE, bool STAGED, typename T>
+template <int S, bool BLUE, bool STAGED, int P2, typename T>
 __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
                                                           int comps_per_block, int vcap,
                                                           const double2 *btab)
@@ -351,13 +370,17 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
   if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
   const bool norot = fabs(r.phi0) < 1e-14;
   const double2 *Bt = BLUE ? btab + P.boff : nullptr;
-  // w e^{i m phi0} phase_m, real part only for m = 0
-  auto cval = [&](int m, double2 v) {
+  // w e^{i m phi0} phase_m (rot = e^{i m phi0}), real part only for m = 0
+  auto cval = [&](int m, double2 v, double2 rot) {
     v = cscale(v, wgt);
     if (m == 0) v.y = 0.;
-    else if (!norot) v = cmul(v, expi_small((double)m * r.phi0));
+    else if (!norot) v = cmul(v, rot);
     return v;
   };
+  const double2 one = make_double2(1., 0.);
+  // rotation steps: e^{i NT phi0} between a thread's bins, e^{i n phi0}
+  const double2 rstep = norot ? one : expi_small((double)NT * r.phi0);
+  const double2 rn = norot ? one : expi_small((double)n * r.phi0);
 
   // bins: even n -> C_0..C_N, odd n -> C_0..C_{(n-1)/2}
   const int nbins = even ? N + 1 : (n + 1) / 2;
@@ -394,14 +417,19 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
                       ? a.phase[phase_index(comp, mn, slot, hemi, a.nslots, a.nmb)]
                       : make_double2(0., 0.);
         }
+        // e^{i k phi0} for this thread's bins; e^{i (n-k) phi0} from it
+        double2 rk = norot ? one : expi_small((double)base * r.phi0);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int k = base + u * NT, mn = (k == 0) ? n : n - k;
-          if (k >= nbins) continue;
-          double2 sum = make_double2(0., 0.);
-          if (k <= mmax) sum = cval(k, vp[u]);
-          if (mn <= mmax) sum = cadd(sum, conj(cval(mn, vn[u])));
-          put_bin(k, sum, true);
+          if (k < nbins) {
+            const double2 rmn = (k == 0) ? rn : cmul(rn, conj(rk));
+            double2 sum = make_double2(0., 0.);
+            if (k <= mmax) sum = cval(k, vp[u], rk);
+            if (mn <= mmax) sum = cadd(sum, conj(cval(mn, vn[u], rmn)));
+            put_bin(k, sum, true);
+          }
+          rk = cmul(rk, rstep);
         }
       }
       __syncthreads();
@@ -410,9 +438,22 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
       // with G lanes in a fixed order
       for (int m0 = 0; m0 <= mmax; m0 += vcap) {
         const int m1 = min(mmax + 1, m0 + vcap);
-        batched<NT, 8>(m0, m1,
-            [&](int m) { return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)]; },
-            [&](int m, double2 v) { V[m - m0] = cval(m, v); });
+        for (int base = m0 + tid; base < m1; base += NT * 8) {
+          double2 v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int m = base + u * NT;
+            v[u] = (m < m1) ? a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)]
+                            : make_double2(0., 0.);
+          }
+          double2 rot = norot ? one : expi_small((double)base * r.phi0);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int m = base + u * NT;
+            if (m < m1) V[m - m0] = cval(m, v[u], rot);
+            rot = cmul(rot, rstep);
+          }
+        }
         __syncthreads();
         for (int kb = 0; kb < nbins; kb += bins_per_round) {
           const int k = kb + tid / G, part = tid % G;
@@ -439,23 +480,26 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
       }
     }
   // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
+    //    (e^{2 pi i (N-k)/n} = -conj(e^{2 pi i k/n}) since n = 2N)
     if (even) {
+      const double tsc = 2.0 / n;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
         const double2 ck = buf[k], ckp = (k == 0) ? nyq : buf[kp];
+        const double2 tw = expipi((double)k * tsc);
         const double2 e = cadd(ck, conj(ckp));
-        const double2 o = cmul(csub(ck, conj(ckp)), expipi(2.0 * k / n));
+        const double2 o = cmul(csub(ck, conj(ckp)), tw);
         buf[k] = make_double2(e.x - o.y, e.y + o.x);
         if (kp < N && kp != k) {
           const double2 e2 = cadd(ckp, conj(ck));
-          const double2 o2 = cmul(csub(ckp, conj(ck)), expipi(2.0 * kp / n));
+          const double2 o2 = cmul(csub(ckp, conj(ck)), make_double2(-tw.x, tw.y));
           buf[kp] = make_double2(e2.x - o2.y, e2.y + o2.x);
         }
       }
       __syncthreads();
     }
     // 3. inverse FFT of length N
-    ring_dft<NT, E, BLUE>(buf, P, +1, Bt);
+    ring_dft<NT, E, BLUE, P2>(buf, P, +1, Bt);
     // 4. store
     void *mp = a.maps.p[comp];
     const bool add = (a.flags & SHARP_ADD) != 0;
@@ -478,7 +522,7 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
 // ===========================================================================
 // analysis: ring -> phase
 // ===========================================================================
-template <int S, bool BLUE, typename T>
+template <int S, bool BLUE, int P2, typename T>
 __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
                                                           int comps_per_block, int,
                                                           const double2 *btab)
@@ -498,13 +542,15 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(Ring
   const bool norot = fabs(r.phi0) < 1e-14;
 
   const double2 *Bt = BLUE ? btab + P.boff : nullptr;
+  const double2 one = make_double2(1., 0.);
+  const double2 rstep = norot ? one : expi_small((double)NT * (-r.phi0));
 
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
     const void *mp = a.maps.p[comp];
     // 1. load: even n -> z_j = f_2j + i f_2j+1, odd n -> z_j = f_j
-    batched<NT, 8>(0, N,
+    batched<NT, (P2 == 2) ? 4 : 8>(0, N,
         [&](int j) {
           if (even) {
             const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
@@ -516,9 +562,11 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(Ring
         [&](int j, double2 v) { buf[j] = v; });
     __syncthreads();
     // 2. forward FFT
-    ring_dft<NT, E, BLUE>(buf, P, -1, Bt);
+    ring_dft<NT, E, BLUE, P2>(buf, P, -1, Bt);
     // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
+    //    (e^{-2 pi i (N-k)/n} = -e^{2 pi i k/n} since n = 2N)
     if (even) {
+      const double tsc = 2.0 / n;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
         if (k == 0) {
@@ -526,13 +574,15 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(Ring
           buf[0] = make_double2(z0.x + z0.y, 0.);
           nyq = make_double2(z0.x - z0.y, 0.);
         } else {
+          const double2 tw = expipi((double)k * tsc);
           const double2 zk = buf[k], zkp = buf[kp];
           const double2 s1 = cscale(cadd(zk, conj(zkp)), 0.5);
-          const double2 d1 = cmul(cscale(csub(zk, conj(zkp)), 0.5), expipi(-2.0 * k / n));
+          const double2 d1 = cmul(cscale(csub(zk, conj(zkp)), 0.5), conj(tw));
           buf[k] = make_double2(s1.x + d1.y, s1.y - d1.x);
           if (kp != k) {
             const double2 s2 = cscale(cadd(zkp, conj(zk)), 0.5);
-            const double2 d2 = cmul(cscale(csub(zkp, conj(zk)), 0.5), expipi(-2.0 * kp / n));
+            const double2 d2 = cmul(cscale(csub(zkp, conj(zk)), 0.5),
+                                    make_double2(-tw.x, -tw.y));
             buf[kp] = make_double2(s2.x + d2.y, s2.y - d2.x);
           }
         }
@@ -540,13 +590,19 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(Ring
       __syncthreads();
     }
     // 4. phases
-    for (int m = tid; m <= mmax; m += NT) {
+    double2 rot = one;
+    int step = 0;
+    for (int m = tid; m <= mmax; m += NT, ++step) {
       const int k = m % n;
       double2 v;
       if (even) v = (k < N) ? buf[k] : ((k == N) ? nyq : conj(buf[n - k]));
       else v = buf[k];
       v = cscale(v, wgt);
-      if (!norot) v = cmul(v, expi_small((double)m * (-r.phi0)));
+      if (!norot) {
+        // e^{-i m phi0}: one sincos every 8 steps, complex products between
+        rot = ((step & 7) == 0) ? expi_small((double)m * (-r.phi0)) : cmul(rot, rstep);
+        v = cmul(v, rot);
+      }
       a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] = v;
     }
     __syncthreads();
@@ -583,6 +639,7 @@ __global__ void __launch_bounds__(kThreads<S>) k_bluestein_table(const RingFftPl
   const RingFftPlan &P = plans[blockIdx.x];
   bluestein_kernel<NT, E>(smem, P, -1);
   for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = smem[j];
+  for (int j = threadIdx.x; j < P.N; j += NT) btab[P.boff + P.M + j] = chirp(j, P.N, -1);
 }
 
 // raises a kernel's dynamic LDS limit once per new maximum (callers hold the
@@ -598,7 +655,7 @@ template <typename K> static void set_lds(K kern, size_t lds)
 
 static size_t plan_need(const RingFftPlan &p) { return p.bluestein ? p.M : p.N; }
 
-template <int S>
+template <int S, int P2>
 static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
                          const RingFftPlan *plans, const RingFftPlan *hplans, int nplans,
                          const double2 *btab, hipStream_t st)
@@ -629,57 +686,70 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   };
 #define SHP_SYN(B, ST)                                                     \
   do {                                                                     \
-    if (a.dp) go(k_phase2ring_fft<S, B, ST, double>);                      \
-    else go(k_phase2ring_fft<S, B, ST, float>);                            \
+    if (a.dp) go(k_phase2ring_fft<S, B, ST, P2, double>);                  \
+    else go(k_phase2ring_fft<S, B, ST, P2, float>);                        \
   } while (0)
   if (synth) {
     if (blue) { if (staged) SHP_SYN(true, true); else SHP_SYN(true, false); }
     else { if (staged) SHP_SYN(false, true); else SHP_SYN(false, false); }
   } else {
     if (blue) {
-      if (a.dp) go(k_ring2phase_fft<S, true, double>);
-      else go(k_ring2phase_fft<S, true, float>);
+      if (a.dp) go(k_ring2phase_fft<S, true, P2, double>);
+      else go(k_ring2phase_fft<S, true, P2, float>);
     } else {
-      if (a.dp) go(k_ring2phase_fft<S, false, double>);
-      else go(k_ring2phase_fft<S, false, float>);
+      if (a.dp) go(k_ring2phase_fft<S, false, P2, double>);
+      else go(k_ring2phase_fft<S, false, P2, float>);
     }
   }
 #undef SHP_SYN
   check_launch(synth ? "phase2ring_fft" : "ring2phase_fft");
 }
 
-template <int S>
+template <int S, int P2>
 static void launch_class(bool synth, bool blue, const RingArgs &a, const RingFftPlan *plans,
                          const RingFftPlan *hplans, int nplans, const double2 *btab,
                          hipStream_t st)
 {
   if (!synth) {
-    launch_range<S>(false, blue, false, a, plans, hplans, nplans, btab, st);
+    launch_range<S, P2>(false, blue, false, a, plans, hplans, nplans, btab, st);
     return;
   }
   // plans are sorted by decreasing n: rings with n > mmax fold directly
   int k = 0;
   while (k < nplans && hplans[k].n > a.mmax) ++k;
-  launch_range<S>(true, blue, false, a, plans, hplans, k, btab, st);
-  launch_range<S>(true, blue, true, a, plans + k, hplans + k, nplans - k, btab, st);
+  launch_range<S, P2>(true, blue, false, a, plans, hplans, k, btab, st);
+  launch_range<S, P2>(true, blue, true, a, plans + k, hplans + k, nplans - k, btab, st);
+}
+
+template <int P2>
+static void launch_kind(int c, bool synth, bool blue, const RingArgs &a,
+                        const RingFftPlan *p, const RingFftPlan *hp, int np,
+                        const double2 *btab, hipStream_t st)
+{
+  switch (c) {
+  case 0: launch_class<512, P2>(synth, blue, a, p, hp, np, btab, st); break;
+  case 1: launch_class<1024, P2>(synth, blue, a, p, hp, np, btab, st); break;
+  case 2: launch_class<2048, P2>(synth, blue, a, p, hp, np, btab, st); break;
+  case 3: launch_class<4096, P2>(synth, blue, a, p, hp, np, btab, st); break;
+  case 4: launch_class<8192, P2>(synth, blue, a, p, hp, np, btab, st); break;
+  default: break;
+  }
 }
 
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st)
 {
   // largest rings first: they take longest
   for (int c = kNumFftClasses - 1; c >= 0; --c) {
-    for (int blue = 1; blue >= 0; --blue) {
-      const int idx = 2 * c + blue;
+    for (int kind = kFftKinds - 1; kind >= 0; --kind) {
+      const int idx = kFftKinds * c + kind;
       if (fc.count[idx] == 0) continue;
       const RingFftPlan *p = fc.plans[idx], *hp = fc.hplans[idx];
       const int np = fc.count[idx];
-      switch (c) {
-      case 0: launch_class<512>(synth, blue, a, p, hp, np, fc.btab, st); break;
-      case 1: launch_class<1024>(synth, blue, a, p, hp, np, fc.btab, st); break;
-      case 2: launch_class<2048>(synth, blue, a, p, hp, np, fc.btab, st); break;
-      case 3: launch_class<4096>(synth, blue, a, p, hp, np, fc.btab, st); break;
-      case 4: launch_class<8192>(synth, blue, a, p, hp, np, fc.btab, st); break;
-      default: break;
+      const bool blue = (kind & 1) != 0;
+      switch (kind >> 1) {
+      case 2: launch_kind<2>(c, synth, blue, a, p, hp, np, fc.btab, st); break;
+      case 1: launch_kind<1>(c, synth, blue, a, p, hp, np, fc.btab, st); break;
+      default: launch_kind<0>(c, synth, blue, a, p, hp, np, fc.btab, st); break;
       }
     }
   }
@@ -701,7 +771,7 @@ static void table_class(const RingFftPlan *plans, const RingFftPlan *hplans, int
 void launch_bluestein_table(const FftClasses &u, double2 *btab, hipStream_t st)
 {
   for (int c = 0; c < kNumFftClasses; ++c) {
-    const int idx = 2 * c + 1;
+    const int idx = kFftKinds * c + 1;   // the table list holds every Bluestein plan here
     const RingFftPlan *p = u.plans[idx], *hp = u.hplans[idx];
     const int np = u.count[idx];
     switch (c) {
