@@ -57,8 +57,14 @@ constexpr int kMixMaxT0 = 16;   // spin 0: 2 real columns per transform
 constexpr int kMixMaxTS = 8;    // spin s: 4 real columns per transform
 // analysis: wavefronts per workgroup (spin 0 / spin s); a workgroup sums the
 // ring pairs of one group of 64*waves slots
-constexpr int kMixWaves0 = 8;
-constexpr int kMixWavesS = 4;
+#ifndef SHP_MIX_WAVES0
+#define SHP_MIX_WAVES0 4
+#endif
+#ifndef SHP_MIX_WAVESS
+#define SHP_MIX_WAVESS 4
+#endif
+constexpr int kMixWaves0 = SHP_MIX_WAVES0;
+constexpr int kMixWavesS = SHP_MIX_WAVESS;
 constexpr int mix_group_slots(int spin) { return 64 * (spin == 0 ? kMixWaves0 : kMixWavesS); }
 void launch_msynth(const LegArgs &a, hipStream_t st);
 // f64: the FP64 form of the same kernels (FP64 matrix cores, FP64 partial

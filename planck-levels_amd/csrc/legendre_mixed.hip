@@ -652,6 +652,15 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 // ===========================================================================
 // analysis
 // ===========================================================================
+// the double held by lane `src` (a compile-time constant after unrolling) in
+// every lane, through two v_readlane_b32: a wave-uniform (scalar) value
+__device__ inline double bcast_d(double v, int src)
+{
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 constexpr int kLS = 33;    // LDS row stride (floats) of the lambda slices: odd,
                            // so row-wise stores and the transposed MFMA loads
                            // hit distinct banks
@@ -738,24 +747,48 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   const double mf = a.mfac[m];
   lam2 *= (m & 1) ? -mf : mf;
   normalize(lam2, scale, kFTol);
+  // a ring pair beyond its m limit contributes nothing: zero lambda, in
+  // range, so it neither contributes nor holds back the mode switches
+  if (!active) { lam2 = 0.; scale = 0; }
   // 0: advancing below the IEEE range, 1: scaled, 2: unscaled, 3: idle
   int mode = !wave_any(active) ? 3 : (wave_all(scale < 1) ? 0 : 1);
+  mode = __builtin_amdgcn_readfirstlane(mode);
   double cf = corfac(scale);
   int l = m;
   T *mine = lds[w];
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww];
-  // recurrence coefficients of the current 8 l; the next 8 are fetched
-  // after the first two-step of each group of 8 (see the synthesis kernels)
+  // coefficients rf[lc .. lc+31] of a chunk as 64 doubles, one per lane,
+  // loaded a chunk ahead and broadcast with readlane in the unscaled chunks
+  const double *rfd = reinterpret_cast<const double *>(rf);
+  const int rdmax = 2 * lpad + 1;
+  double cq = rfd[min(2 * m + lane, rdmax)];
   double2 rc[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
 
   const int nchunks = (lmax - m + 32) / 32;
   for (int c = 0; c < nchunks; ++c) {
     const int lc = m + 32 * c;
+    const double cqn = rfd[min(2 * (lc + 32) + lane, rdmax)];
     bool any = false;
+    if (mode == 2 && lc + 31 <= lmax) {
+      // unscaled and whole: no checks; lambda of every lane stays in range
+      any = true;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const double o0 = lam2;
+        lam1 = (cth * lam2) * bcast_d(cq, 4 * q) - lam1 * bcast_d(cq, 4 * q + 1);
+        const double o1 = lam1;
+        lam2 = (cth * lam1) * bcast_d(cq, 4 * q + 2) - lam2 * bcast_d(cq, 4 * q + 3);
+        mine[lane * kLS + 2 * q] = (T)o0;
+        mine[lane * kLS + 2 * q + 1] = (T)o1;
+      }
+      l += 32;
+    } else {
+    // recurrence coefficients of the current 8 l; the next 8 are fetched
+    // after the first two-step of each group of 8 (see the synthesis kernels)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
 #pragma unroll
     for (int g8 = 0; g8 < 4; ++g8) {
       const int lb = l;
@@ -802,6 +835,9 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int k = 0; k < 8; ++k) rc[k] = rn[k];
     }
+    mode = __builtin_amdgcn_readfirstlane(mode);
+    }
+    cq = cqn;
     // the slice is read back by the same wavefront: LDS operations of one
     // wavefront complete in order, the fence keeps the compiler from moving
     // the loads above the stores
@@ -898,23 +934,58 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   double rec1p = 0., rec1m = 0., rec2p, rec2m;
   int scalep, scalem;
   spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep, scalem);
+  // a ring pair beyond its m limit: zero lambda, in range (as for spin 0)
+  if (!active) { rec2p = rec2m = 0.; scalep = scalem = 0; }
   int mode = !wave_any(active) ? 3 : (wave_all(scalep < 1 && scalem < 1) ? 0 : 1);
+  mode = __builtin_amdgcn_readfirstlane(mode);
   double cfp = corfac(scalep), cfm = corfac(scalem);
   int l = st.mhi;
   T *mw = lds[w][0], *mx = lds[w][1];
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
-  // coefficients fx(l+1..l+4) of the current two two-steps, prefetched as in
-  // the synthesis kernels
+  // coefficients fx(lc+1 .. lc+32) of a chunk as 96 doubles, two per lane
+  // in lanes 0..47, loaded a chunk ahead and broadcast with readlane in the
+  // unscaled chunks
+  const int fdmax = 3 * lpad + 2;
+  auto chunk_coef = [&](int lc, double &c0, double &c1) {
+    const int b = 3 * (lc + 1) + 2 * min(lane, 47);
+    c0 = fxb[min(b, fdmax)];
+    c1 = fxb[min(b + 1, fdmax)];
+  };
+  double cqa, cqb;
+  chunk_coef(st.mhi, cqa, cqb);
+  auto cf_at = [&](int i) { return bcast_d((i & 1) ? cqb : cqa, i >> 1); };
   Fx fc[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
 
   const int nchunks = (lmax - st.mhi + 32) / 32;
   for (int c = 0; c < nchunks; ++c) {
     const int lc = st.mhi + 32 * c;
+    double cqna, cqnb;
+    chunk_coef(lc + 32, cqna, cqnb);
     bool any = false;
+    if (mode == 2 && lc + 31 <= lmax) {
+      // unscaled and whole: no checks; lambda of every lane stays in range
+      any = true;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const Fx f1{cf_at(6 * j), cf_at(6 * j + 1), cf_at(6 * j + 2)};
+        const Fx f2{cf_at(6 * j + 3), cf_at(6 * j + 4), cf_at(6 * j + 5)};
+        const double rp0 = rec2p, rm0 = rec2m;
+        rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
+        const double rp1 = rec1p, rm1 = rec1m;
+        rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
+        mw[lane * kLS + 2 * j] = (T)(rp0 + rm0);
+        mw[lane * kLS + 2 * j + 1] = (T)(rp1 + rm1);
+        mx[lane * kLS + 2 * j] = (T)(rm0 - rp0);
+        mx[lane * kLS + 2 * j + 1] = (T)(rm1 - rp1);
+      }
+      l += 32;
+    } else {
+    // coefficients fx(l+1..l+4) of the current two two-steps, prefetched as
+    // in the synthesis kernels
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
 #pragma unroll
     for (int g4 = 0; g4 < 8; ++g4) {
       const int lb = l;
@@ -972,6 +1043,10 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int k = 0; k < 4; ++k) fc[k] = fn[k];
     }
+    mode = __builtin_amdgcn_readfirstlane(mode);
+    }
+    cqa = cqna;
+    cqb = cqnb;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -23,7 +23,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsharp_hip.so")
+# SHARP_HIP_LIBRARY names another build of the library (e.g. one made with
+# different build-time options); default: the in-tree build
+LIB_PATH = os.environ.get("SHARP_HIP_LIBRARY") or os.path.join(HERE, "libsharp_hip.so")
 
 # job types and flags (libsharp/sharp_lowlevel.h:162-191, include/sharp_hip.h)
 MAP2ALM, ALM2MAP, Yt, WY, ALM2MAP_DERIV1 = 0, 1, 2, 3, 4
