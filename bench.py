@@ -190,7 +190,7 @@ def measured_traffic(workload, stage):
     return ent["read_bytes"] + ent["write_bytes"]
 
 
-def roofline(stats, mixed, nt, workload=None):
+def roofline(stats, mixed, nt, workload=None, bf16=False):
     """Roofline object of the slowest stage.  Legendre stages carry libsharp's
     flop model per launch (c0 + c1*ntrans per triple: c0 the FP64 recurrence,
     c1*ntrans the contraction); ring stages carry compulsory HBM bytes.
@@ -203,6 +203,19 @@ def roofline(stats, mixed, nt, workload=None):
         if mixed:
             c = contraction_share(dom["name"], nt)
             ach_c = ach * c
+            if bf16:
+                # six BF16 products per FP32 product: the matrix cores run 6x
+                # the model's contraction flops, priced at the BF16 peak
+                return {"kernel": dom["name"], "bound": "mfma", "achieved": 6 * ach_c / 1e12,
+                        "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
+                        "frac": 6 * ach_c / PEAK["bf16"], "traffic": traffic,
+                        "avg_launch_ms": avg_s * 1e3,
+                        "model_contraction_tflops": ach_c / 1e12,
+                        "fp64_recurrence_tflops": ach * (1 - c) / 1e12,
+                        "note": "achieved = 6 x the contraction flops of libsharp's flop "
+                                "model (six BF16 piece products per FP32 product) over "
+                                "the dense BF16 MFMA peak; the FP64 recurrence share "
+                                "(c0 per triple) is given separately"}
             return {"kernel": dom["name"], "bound": "mfma", "achieved": ach_c / 1e12,
                     "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                     "frac": ach_c / PEAK["fp32"], "traffic": traffic,
@@ -230,8 +243,9 @@ def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
     hT = [t.cpu().numpy() for t in T]
     hEB = [t.cpu().numpy() for t in EB]
     res = {}
+    form_flags = {"mixed": S.MIXED_FP32, "bf16x3": S.MIXED_BF16X3, "fp64": 0}
     for form in forms:
-        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+        flags = S.DP | form_flags[form]
         map_t = [np.zeros(npix) for _ in range(batch)]
         map_qu = [np.zeros(npix) for _ in range(2 * batch)]
         out_t = [np.zeros_like(hT[0]) for _ in range(batch)]
@@ -278,8 +292,8 @@ def smoothing_workload(S, torch, dev, args):
     torch.cuda.synchronize()
     del T, E, B, EB
     res = {}
-    for form in ("mixed", "fp64"):
-        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+    for form in ("bf16x3", "fp64"):
+        flags = S.DP | (S.MIXED_BF16X3 if form == "bf16x3" else 0)
         wt = [x.clone() for x in m_t]
         wqu = [x.clone() for x in m_qu]
         at = [torch.zeros(S.nalm_triangular(lmax), dtype=torch.complex128, device=dev)
@@ -299,23 +313,24 @@ def smoothing_workload(S, torch, dev, args):
         val = batch * args.steps / dt
         show(form, stats, ms, val)
         res[form] = {"value": val, "ms_per_step": ms, "maps": wt + wqu,
-                     "roofline": roofline(stats, form == "mixed", batch)}
+                     "roofline": roofline(stats, form != "fp64", batch,
+                                          bf16=form == "bf16x3")}
 
     def rel(x, y):
         return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
-    err = max(rel(x, y) for x, y in zip(res["mixed"]["maps"], res["fp64"]["maps"]))
+    err = max(rel(x, y) for x, y in zip(res["bf16x3"]["maps"], res["fp64"]["maps"]))
     out = {
         "metric": "smoothing_cxx TQU maps/s, Nside %d lmax %d, 3 iterations, 5' beam" % (nside, lmax),
-        "value": res["mixed"]["value"], "unit": "TQU maps smoothed/s", "n_gpus": 1,
+        "value": res["bf16x3"]["value"], "unit": "TQU maps smoothed/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": res["mixed"]["ms_per_step"], "higher_is_better": True,
+        "ms_per_step": res["bf16x3"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-mfma contraction, f64 recurrence/FFT",
+        "dtype": "bf16-mfma contraction on 3-piece splits of f32, f64 recurrence/FFT",
         "data": "synthetic: TQU maps synthesised from the seeded a_lm of the headline workload",
         "config": {"workload": "smoothing_cxx core (map2alm_iter, smoothWithGauss, alm2map) "
                                "of 8 TQU maps", "nside": nside, "lmax": lmax,
                    "batch": batch, "iterations": niter, "fwhm_arcmin": 5.0},
-        "roofline": res["mixed"]["roofline"],
+        "roofline": res["bf16x3"]["roofline"],
         "fp64_form": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
                       "roofline": res["fp64"]["roofline"]},
         "mixed_vs_fp64_max_rel_l2": err,
@@ -411,9 +426,10 @@ def main():
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=None)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--form", default="mixed", choices=["mixed", "fp64"],
-                    help="form whose throughput is `value`; the other one is "
-                         "timed beside it")
+    ap.add_argument("--form", default="bf16x3", choices=["mixed", "bf16x3", "fp64"],
+                    help="form whose throughput is `value` (mixed: FP32 matrix "
+                         "cores; bf16x3: BF16 matrix cores on three-piece splits); "
+                         "the other ones are timed beside it")
     ap.add_argument("--only", action="store_true",
                     help="time only --form (no comparison run)")
     ap.add_argument("--seed", type=int, default=1234)
@@ -467,10 +483,12 @@ def main():
         log("copy probe: 1 GiB read + 1 GiB written")
         del src, dst
 
-    forms = [args.form] if args.only else [args.form, "fp64" if args.form == "mixed" else "mixed"]
+    forms = [args.form] if args.only else \
+        [args.form] + [f for f in ("mixed", "bf16x3", "fp64") if f != args.form]
+    form_flags = {"mixed": S.MIXED_FP32, "bf16x3": S.MIXED_BF16X3, "fp64": 0}
     res = {}
     for form in forms:
-        flags = S.DP | (S.MIXED_FP32 if form == "mixed" else 0)
+        flags = S.DP | form_flags[form]
         map_t = [zeros() for _ in range(batch)]
         map_qu = [zeros() for _ in range(2 * batch)]
         out_t = [torch.zeros_like(T[0]) for _ in range(batch)]
@@ -488,18 +506,23 @@ def main():
         val = batch * args.steps / dt
         show(form, stats, ms, val)
         res[form] = {"value": val, "ms_per_step": ms, "stats": stats,
-                     "roofline": roofline(stats, form == "mixed", batch,
-                                          "nside%d_lmax%d_tqu%d_%s" % (nside, lmax, batch, form)),
+                     "roofline": roofline(stats, form != "fp64", batch,
+                                          "nside%d_lmax%d_tqu%d_%s" % (nside, lmax, batch, form),
+                                          bf16=form == "bf16x3"),
                      "maps": map_t + map_qu, "alms": out_t + out_eb}
 
-    # mixed vs FP64 on the GPU: maps after alm2map, a_lm after the round trip
-    mix_err = None
-    if "mixed" in res and "fp64" in res:
-        def rel(x, y):
-            return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
-        mm, fm = res["mixed"]["maps"], res["fp64"]["maps"]
-        ma, fa = res["mixed"]["alms"], res["fp64"]["alms"]
-        mix_err = {
+    # mixed forms vs FP64 on the GPU: maps after alm2map, a_lm after the
+    # round trip
+    def rel(x, y):
+        return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+
+    errs = {}
+    for form in res:
+        if form == "fp64" or "fp64" not in res:
+            continue
+        mm, fm = res[form]["maps"], res["fp64"]["maps"]
+        ma, fa = res[form]["alms"], res["fp64"]["alms"]
+        e = {
             "maps_T_max_rel_l2": max(rel(mm[i], fm[i]) for i in range(batch)),
             "maps_QU_max_rel_l2": max(rel(mm[batch + i], fm[batch + i])
                                       for i in range(2 * batch)),
@@ -507,9 +530,11 @@ def main():
             "roundtrip_alm_EB_max_rel_l2": max(rel(ma[batch + i], fa[batch + i])
                                                for i in range(2 * batch)),
         }
-        mix_err["max"] = max(mix_err.values())
-        mix_err["bound"] = 1e-6
-        log("mixed vs FP64:", mix_err)
+        e["max"] = max(e.values())
+        e["bound"] = 1e-6
+        errs[form] = e
+        log("%s vs FP64:" % form, e)
+    mix_err = errs.get(args.form)
 
     host = None
     if args.host_path:
@@ -543,7 +568,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma contraction, f64 recurrence/FFT" if args.form == "mixed" else "f64",
+        "dtype": {"mixed": "f32-mfma contraction, f64 recurrence/FFT",
+                  "bf16x3": "bf16-mfma contraction on 3-piece splits of f32 (6 products, "
+                            "f32 accumulation), f64 recurrence/FFT",
+                  "fp64": "f64"}[args.form],
         "data": "synthetic: seeded Gaussian a_lm, C_l^TT=1/(l(l+1)+1) x 5' beam, "
                 "EE=0.1TT, BB=0.01TT, TE corr 0.5",
         "config": {"workload": "HEALPix Nside %d, lmax %d, TQU, batch %d, "
@@ -557,6 +585,8 @@ def main():
             out[form + "_form"] = {"value": res[form]["value"],
                                    "ms_per_step": res[form]["ms_per_step"],
                                    "roofline": res[form]["roofline"]}
+            if form in errs:
+                out[form + "_form"]["vs_fp64"] = errs[form]
     if mix_err is not None:
         out["mixed_vs_fp64"] = mix_err
     if host is not None:

@@ -26,6 +26,11 @@ extern "C" {
    [rings x l] . [l x maps] contraction on the FP32 matrix cores with FP32
    accumulation; its outputs stay within 1e-6 relative L2 of the FP64 form. */
 #define SHARP_HIP_MIXED_FP32    (1<<24)
+/* As SHARP_HIP_MIXED_FP32, with the contraction on the BF16 matrix cores:
+   every FP32 operand is split exactly into three BF16 pieces and the six
+   piece products of order >= 2^-16 are accumulated in FP32 (the dropped ones
+   are below 2^-23 of each product).  Not for batches. */
+#define SHARP_HIP_MIXED_BF16X3  (1<<25)
 
 /* Device-resident form of sharp_execute (libsharp/sharp_lowlevel.h:222-224).
    *opcnt as in sharp_execute; no timing is taken. */

@@ -633,6 +633,8 @@ static Job make_job(int type, int spin, const sharp_geom_info *g,
     seen[mc] = 1;
   }
   j.mmax = a->nm - 1;
+  if ((flags & SHARP_HIP_MIXED_FP32) && (flags & SHARP_HIP_MIXED_BF16X3))
+    SHP_FAIL("SHARP_HIP_MIXED_FP32 and SHARP_HIP_MIXED_BF16X3 exclude each other");
   if (j.lmax < spin) SHP_FAIL("incorrect l_max: must be >= spin");
   if (j.lmax < j.mmax) SHP_FAIL("incorrect l_max: must be >= m_max");
   if (j.flags & SHARP_NO_FFT) {
@@ -707,6 +709,17 @@ static size_t almtmp_elems(const Job &j)
   return ((size_t)tri_ofs(j.mmax + 1, j.lmax) + kAlmSlackRows) * j.ntrans * j.nalm;
 }
 
+// the mixed forms: FP32 matrix cores, or BF16 matrix cores on three-piece
+// splits of the FP32 operands
+static bool is_mixed(int flags)
+{
+  return (flags & (SHARP_HIP_MIXED_FP32 | SHARP_HIP_MIXED_BF16X3)) != 0;
+}
+static int mixed_contr(int flags)
+{
+  return (flags & SHARP_HIP_MIXED_BF16X3) ? kContrBf16x3 : kContrF32;
+}
+
 // norm_l * f_l on the device for a job with per-l factors, copied on `st`
 // through pinned memory (reused once the previous copy out of it has ended)
 static double *upload_lfac(Ctx &c, const Job &j, hipStream_t st)
@@ -759,6 +772,8 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   aa.mvstart_by_m = d_mvs;
   aa.norm_l = j.lfac ? upload_lfac(c, j, st) : d_norm;
   aa.almtmp = almtmp;
+  // the mixed syntheses read FP32 a_lm
+  aa.tmp_f32 = (is_mixed(j.flags) && j.type == SHARP_ALM2MAP) ? 1 : 0;
   for (int i = 0; i < ncomp_alm; ++i) aa.alms.p[i] = dalm[i];
   aa.part = nullptr;
   aa.gmax = ml.gmax;
@@ -895,11 +910,17 @@ static void stage_legendre_fp64(Ctx &c, JobState &s, hipStream_t st)
   }
 }
 
+
 static const char *mixed_name(const Job &j)
 {
-  if (j.type == SHARP_ALM2MAP)
-    return j.deriv ? "mixed_synth_deriv1" : (j.spin == 0 ? "mixed_synth_spin0" : "mixed_synth_spin");
-  return j.spin == 0 ? "mixed_anal_spin0" : "mixed_anal_spin";
+  const bool bf = (j.flags & SHARP_HIP_MIXED_BF16X3) != 0;
+  if (j.type == SHARP_ALM2MAP) {
+    if (j.deriv) return bf ? "bf16x3_synth_deriv1" : "mixed_synth_deriv1";
+    if (j.spin == 0) return bf ? "bf16x3_synth_spin0" : "mixed_synth_spin0";
+    return bf ? "bf16x3_synth_spin" : "mixed_synth_spin";
+  }
+  if (j.spin == 0) return bf ? "bf16x3_anal_spin0" : "mixed_anal_spin0";
+  return bf ? "bf16x3_anal_spin" : "mixed_anal_spin";
 }
 
 // SHARP_HIP_FP64_ANALYSIS=reduce selects the FP64 analysis kernels that sum
@@ -917,7 +938,7 @@ static bool fp64_anal_reduce()
 static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
 {
   const Job &j = s.j;
-  const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
+  const bool mixed = is_mixed(j.flags);
   if (!mixed && fp64_anal_reduce()) {
     stage_legendre_fp64(c, s, st);
     timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
@@ -939,7 +960,7 @@ static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
     la.part_ld = 2 * npc;
     la.part_col0 = 0;
     timed(c, st, name, s.triples * (s.c0 + s.c1 * la.ntl), 0.,
-          [&] { launch_manal(la, !mixed, st); });
+          [&] { launch_manal(la, mixed ? mixed_contr(j.flags) : kContrF64, st); });
     s.aa.part = la.part;
     s.aa.part_f64 = mixed ? 0 : 1;
     s.aa.part_ld = la.part_ld;
@@ -960,7 +981,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   double2 *almtmp = static_cast<double2 *>(c.almtmp.get(almtmp_elems(j) * sizeof(double2)));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
 
-  const bool mixed = (j.flags & SHARP_HIP_MIXED_FP32) != 0;
+  const bool mixed = is_mixed(j.flags);
   if (j.type != SHARP_ALM2MAP) {
     stage_rings_to_phase(c, s, st);
     stage_analysis(c, s, c.part, st);
@@ -978,7 +999,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
       la.t0 = t;
       la.ntl = std::min(tmax, j.ntrans - t);
       timed(c, st, mixed_name(j), s.triples * (s.c0 + s.c1 * la.ntl), 0.,
-            [&] { launch_msynth(la, st); });
+            [&] { launch_msynth(la, mixed_contr(j.flags), st); });
     }
   }
   stage_phase_to_rings(c, s, st);
@@ -1443,7 +1464,9 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
   Ctx &c = context();
   auto b = std::make_unique<BatchPlan>();
   b->device = c.device;
-  b->mixed = (flags & SHARP_HIP_MIXED_FP32) != 0;
+  b->mixed = is_mixed(flags);
+  if (flags & SHARP_HIP_MIXED_BF16X3)
+    SHP_FAIL("batches run the FP32 mixed form (SHARP_HIP_MIXED_FP32) or FP64");
   b->use_graph = use_graph != 0;
   for (int i = 0; i < njobs; ++i) {
     const sharp_hip_job &q = jobs[i];

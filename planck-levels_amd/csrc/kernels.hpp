@@ -66,10 +66,12 @@ constexpr int kMixMaxTS = 8;    // spin s: 4 real columns per transform
 constexpr int kMixWaves0 = SHP_MIX_WAVES0;
 constexpr int kMixWavesS = SHP_MIX_WAVESS;
 constexpr int mix_group_slots(int spin) { return 64 * (spin == 0 ? kMixWaves0 : kMixWavesS); }
-void launch_msynth(const LegArgs &a, hipStream_t st);
-// f64: the FP64 form of the same kernels (FP64 matrix cores, FP64 partial
-// sums; up to 8 spin-0 / 4 spin-s transforms per launch)
-void launch_manal(const LegArgs &a, bool f64, hipStream_t st);
+// contraction of the matrix-core Legendre kernels: FP32 MFMA; FP64 MFMA
+// (the FP64 form: FP64 partial sums, up to 8 spin-0 / 4 spin-s transforms
+// per launch); BF16 MFMA on exact three-piece BF16 splits of FP32 operands
+enum { kContrF32 = 0, kContrF64 = 1, kContrBf16x3 = 2 };
+void launch_msynth(const LegArgs &a, int contr, hipStream_t st);
+void launch_manal(const LegArgs &a, int contr, hipStream_t st);
 constexpr int kF64MaxT0 = 8;
 constexpr int kF64MaxTS = 4;
 
@@ -137,6 +139,7 @@ struct AlmArgs {
   const int64_t *mvstart_by_m;  // [mmax+1]
   const double *norm_l; // [lmax+1]
   double2 *almtmp;
+  int tmp_f32;          // gather: almtmp holds float2 (the mixed syntheses)
   PtrTab alms;          // [ncomp] device pointers (float or double data)
   // analysis from partial sums (mixed forms): when part != nullptr the
   // components [c0, c0+npc) are sum_g part[g] over the groups g with

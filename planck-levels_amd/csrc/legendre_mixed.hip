@@ -87,6 +87,50 @@ __device__ inline void row_transpose4(float v[4])
   v[3] = __uint_as_float(d[1]);
 }
 
+// Exact three-piece BF16 split of FP32 values: x = h + m + l, each piece the
+// top 8 significant bits of what remains (truncation; the last remainder has
+// at most 8 significant bits, so l is exact).  Eight values per operand of
+// v_mfma_f32_16x16x32_bf16.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct Bf3 { bf16x8 h, m, l; };
+__device__ inline Bf3 split3(const float (&v)[8])
+{
+  u32x4 H, M, L;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    unsigned hb[2], mb[2], lb[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const float x = v[2 * p + e];
+      const unsigned xu = __float_as_uint(x);
+      const float r1 = x - __uint_as_float(xu & 0xffff0000u);
+      const unsigned ru = __float_as_uint(r1);
+      const float r2 = r1 - __uint_as_float(ru & 0xffff0000u);
+      hb[e] = xu;
+      mb[e] = ru;
+      lb[e] = __float_as_uint(r2);
+    }
+    // element 2p in the low half of word p, element 2p+1 in the high half
+    H[p] = (hb[1] & 0xffff0000u) | (hb[0] >> 16);
+    M[p] = (mb[1] & 0xffff0000u) | (mb[0] >> 16);
+    L[p] = (lb[1] & 0xffff0000u) | (lb[0] >> 16);
+  }
+  return Bf3{__builtin_bit_cast(bf16x8, H), __builtin_bit_cast(bf16x8, M),
+             __builtin_bit_cast(bf16x8, L)};
+}
+// D += A.B from the six piece products of order >= 2^-16 (the three dropped
+// ones are below 2^-23 of |A||B|), smallest first
+__device__ inline f4 mfma_bf3(const Bf3 &a, const Bf3 &b, f4 d)
+{
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.m, b.m, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.m, b.h, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.m, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, d, 0, 0, 0);
+  return d;
+}
 // Maps a linear workgroup id to (major, minor) so that the nminor workgroups
 // of one major index land on the same XCD (ids are dealt round-robin over 8).
 // False for the padding ids of the last round.
@@ -180,8 +224,8 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
         reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
     const int lpad = lmax + kTabPad;   // last readable entry of the row
     // row l of this m, first column of this launch
-    const double *arow = reinterpret_cast<const double *>(a.alm) +
-                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0) * 2;
+    const float *arow = reinterpret_cast<const float *>(a.alm) +
+                        ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0) * 2;
     int colo[NCB];
     bool colok[NCB];
 #pragma unroll
@@ -199,8 +243,8 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
       boff1[cb] = 2 * kq * 2 * nc + colo[cb];
       boff2[cb] = boff1[cb] + 2 * nc;
     }
-    auto ldB = [&](int l0, double (&b1)[NCB], double (&b2)[NCB]) {
-      const double *row = arow + (int64_t)l0 * 2 * nc;
+    auto ldB = [&](int l0, float (&b1)[NCB], float (&b2)[NCB]) {
+      const float *row = arow + (int64_t)l0 * 2 * nc;
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) {
         b1[cb] = row[boff1[cb]];
@@ -240,7 +284,7 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
       double2 rc[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
-      double bc1[NCB], bc2[NCB];
+      float bc1[NCB], bc2[NCB];
       ldB(l, bc1, bc2);
       auto mfmas = [&](const float (&v1)[4], const float (&v2)[4], const float (&b1)[NCB],
                        const float (&b2)[NCB]) {
@@ -256,7 +300,7 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
       // one block of 8 l with every check (scaled domain, rows past lmax)
       auto general_block = [&]() {
         const int lb = l;
-        double bn1[NCB], bn2[NCB];
+        float bn1[NCB], bn2[NCB];
         ldB(lb + 8, bn1, bn2);
         double2 rn[8];
         float v1[4], v2[4];
@@ -326,7 +370,7 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
 #pragma unroll
           for (int k = 0; k < kFlushBlocks; ++k) {
             const int lb = l;
-            double bn1[NCB], bn2[NCB];
+            float bn1[NCB], bn2[NCB];
             ldB(lb + 8, bn1, bn2);
             double2 rn[8];
             float v1[4], v2[4];
@@ -380,7 +424,9 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
 // ===========================================================================
 // spin s synthesis (and ALM2MAP_DERIV1)
 // ===========================================================================
-template <int NCB, bool DERIV>
+// CT: 0 FP32 matrix cores; 1 BF16 matrix cores on three-piece splits in the
+// unscaled part (groups of 32 l), FP32 for the scaled blocks and the tail
+template <int NCB, bool DERIV, int CT>
 __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 {
   const int lane = threadIdx.x;
@@ -412,8 +458,8 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
     // unclamped: for l <= lpad only
     auto fxu = [&](int l) { return Fx{fxb[3 * l], fxb[3 * l + 1], fxb[3 * l + 2]}; };
     constexpr int NALM = DERIV ? 1 : 2;
-    const double *arow = reinterpret_cast<const double *>(a.alm) +
-                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0 * NALM) * 2;
+    const float *arow = reinterpret_cast<const float *>(a.alm) +
+                        ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0 * NALM) * 2;
     // Column col = 4t+q is (Q.re, Q.im, U.re, U.im)[q] of transform t.  Its
     // B entry is X = (E.re, E.im, B.re, B.im)[q] for lambda_w and
     // Y = (-B.im, B.re, E.im, -E.re)[q] for lambda_x (libsharp/
@@ -446,8 +492,8 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
       xo[cb] += kq * 2 * nc;
       yo[cb] += kq * 2 * nc;
     }
-    auto ldB = [&](int l0, double (&bx)[NCB], double (&by)[NCB]) {
-      const double *row = arow + (int64_t)l0 * 2 * nc;
+    auto ldB = [&](int l0, float (&bx)[NCB], float (&by)[NCB]) {
+      const float *row = arow + (int64_t)l0 * 2 * nc;
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) {
         bx[cb] = row[xo[cb]];
@@ -486,7 +532,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
       Fx fc[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
-      double bcx[NCB], bcy[NCB];
+      float bcx[NCB], bcy[NCB];
       ldB(l, bcx, bcy);
       const bool odd = (kq & 1) != 0;
       auto mfmas = [&](const float (&v1)[4], const float (&v2)[4], const float (&b1)[NCB],
@@ -515,7 +561,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
       // one block of 4 l with every check (scaled domain, rows past lmax)
       auto general_block = [&]() {
         const int lb = l;
-        double bnx[NCB], bny[NCB];
+        float bnx[NCB], bny[NCB];
         ldB(lb + 4, bnx, bny);
         Fx fn[4];
         float v1[4], v2[4];
@@ -574,6 +620,92 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 
       // 1: scaled domain
       while (scaled && l <= lmax) general_block();
+      if constexpr (CT == 1) {
+        // 2 (BF16 splits): whole groups of 32 l.  The K index k = 0..31 of
+        // the products is l0+k; accumulator 1 takes (w, x, w, x, ...) with B
+        // rows (X, Y, X, Y, ...), accumulator 2 the other way round, as in
+        // the FP32 blocks.  Lane n+16kq holds B rows l0+8kq..l0+8kq+7.
+        if (!scaled && l + 31 <= lmax) {
+          flush(tot1, c1);
+          flush(tot2, c2);
+          nblk = 0;
+          int bxo[NCB], byo[NCB];
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            bxo[cb] = xo[cb] - kq * 2 * nc + 8 * kq * 2 * nc;
+            byo[cb] = yo[cb] - kq * 2 * nc + 8 * kq * 2 * nc;
+          }
+          while (l + 31 <= lmax) {
+            const int l0 = l;
+            float b1[NCB][8], b2[NCB][8];
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+              const float *row = arow + (int64_t)(l0 + p) * 2 * nc;
+#pragma unroll
+              for (int cb = 0; cb < NCB; ++cb) {
+                const float bx = xs[cb] * row[bxo[cb]], by = ys[cb] * row[byo[cb]];
+                b1[cb][p] = (p & 1) ? by : bx;
+                b2[cb][p] = (p & 1) ? bx : by;
+              }
+            }
+            float a1[32], a2[32];
+#pragma unroll
+            for (int g4 = 0; g4 < 8; ++g4) {
+              const int lb = l;
+              Fx fn[4];
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const double rp0 = rec2p, rm0 = rec2m;
+                rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+                const double rp1 = rec1p, rm1 = rec1m;
+                rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+                const int k = 4 * g4 + 2 * h;
+                a1[k] = (float)(rp0 + rm0);
+                a2[k] = (float)(rm0 - rp0);
+                a1[k + 1] = (float)(rm1 - rp1);
+                a2[k + 1] = (float)(rp1 + rm1);
+                if (h == 0) {
+                  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) fn[q] = fxu(lb + 5 + q);
+                  __builtin_amdgcn_sched_barrier(0);
+                }
+              }
+#pragma unroll
+              for (int q = 0; q < 4; ++q) fc[q] = fn[q];
+              l += 4;
+            }
+            // ring group r's values at k = 8kq..8kq+7 to lane row kq, slot r
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+              float t1[4] = {a1[p], a1[8 + p], a1[16 + p], a1[24 + p]};
+              float t2[4] = {a2[p], a2[8 + p], a2[16 + p], a2[24 + p]};
+              row_transpose4(t1);
+              row_transpose4(t2);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) { a1[8 * r + p] = t1[r]; a2[8 * r + p] = t2[r]; }
+            }
+            Bf3 B1[NCB], B2[NCB];
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) { B1[cb] = split3(b1[cb]); B2[cb] = split3(b2[cb]); }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              float v1[8], v2[8];
+#pragma unroll
+              for (int p = 0; p < 8; ++p) { v1[p] = a1[8 * r + p]; v2[p] = a2[8 * r + p]; }
+              const Bf3 A1 = split3(v1), A2 = split3(v2);
+              // the group's sum from zero, then one addition to the total
+              // (as the FP32 blocks flush every 32 l)
+#pragma unroll
+              for (int cb = 0; cb < NCB; ++cb) {
+                tot1[cb][r] += mfma_bf3(A1, B1[cb], f4{0.f, 0.f, 0.f, 0.f});
+                tot2[cb][r] += mfma_bf3(A2, B2[cb], f4{0.f, 0.f, 0.f, 0.f});
+              }
+            }
+          }
+          ldB(l, bcx, bcy);
+        }
+      } else
       // 2: unscaled, whole groups of kFlushBlocks blocks, software-pipelined:
       // the matrix products of a block are issued during the FP64
       // recurrence of the next one
@@ -586,7 +718,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 #pragma unroll
           for (int k = 0; k < kFlushBlocks; ++k) {
             const int lb = l;
-            double bnx[NCB], bny[NCB];
+            float bnx[NCB], bny[NCB];
             ldB(lb + 4, bnx, bny);
             Fx fn[4];
             float v1[4], v2[4];
@@ -665,6 +797,23 @@ constexpr int kLS = 33;    // LDS row stride (floats) of the lambda slices: odd,
                            // so row-wise stores and the transposed MFMA loads
                            // hit distinct banks
 constexpr int kTS = 20;    // LDS row stride of the [16 x 16] reduction tiles
+// BF16-split contraction: lambda slices as [32 l][64 ring pairs] rows of kRS
+// floats (kRS = 2 mod 32: the 8-byte reads of 16 lanes on 16 different rows
+// hit distinct banks); the same 2112 floats as the [64][kLS] layout
+constexpr int kRS = 66;
+static_assert(32 * kRS == 64 * kLS, "slice layouts differ in size");
+
+// eight lambda values of row ll, ring pairs c0..c0+7, of a [l][ring] slice
+__device__ inline void slice_row8(const float *sl, int ll, int c0, float (&v)[8])
+{
+  const float2 *p = reinterpret_cast<const float2 *>(sl + ll * kRS + c0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float2 t = p[q];
+    v[2 * q] = t.x;
+    v[2 * q + 1] = t.y;
+  }
+}
 
 // Adds the wavefronts' tiles (in slice 0 of each wavefront) in a fixed order
 // and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm.
@@ -696,10 +845,13 @@ __device__ inline void write_tiles(T *slice, const typename Mfma<T>::acc (&t)[2]
         slice[((cb * 2 + x) * 16 + Mfma<T>::row(q, v)) * kTS + n] = t[x][cb][v];
 }
 
-template <typename T, int NCB>
+// CT: 0 the matrix cores of T (FP32 or FP64); 1 (T = float) the BF16
+// matrix cores on exact three-piece splits of FP32 operands
+template <typename T, int NCB, int CT>
 __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
                                  T (&lds)[kMixWaves0][64 * kLS])
 {
+  static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
   constexpr int W = kMixWaves0;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
@@ -717,26 +869,53 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   T *prow = static_cast<T *>(a.part) + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
             (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
 
-  // B: p1 = N+S and p2 = N-S of ring pairs wbase+4j+kq, column n
-  T bp[2][16][NCB];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int rs = wbase + 4 * j + kq;
+  // B: p1 = N+S and p2 = N-S of a ring pair rs, column n
+  auto bpair = [&](int rs, int cb, double &p1, double &p2) {
     const bool ok = rs < nslots && a.mlim[rs < nslots ? rs : 0] >= m;
     const bool pr = ok && a.ispair[rs] != 0;
-#pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) {
-      const int col = cb * 16 + n;
-      double nv = 0., sv = 0.;
-      if (ok && col < ncol) {
-        const int comp = a.t0 + (col >> 1), part = col & 1;
-        nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
-        if (pr) sv = ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
-      }
-      bp[0][j][cb] = (T)(nv + sv);
-      bp[1][j][cb] = (T)(nv - sv);
+    const int col = cb * 16 + n;
+    double nv = 0., sv = 0.;
+    if (ok && col < ncol) {
+      const int comp = a.t0 + (col >> 1), part = col & 1;
+      nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
+      if (pr) sv = ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
     }
+    p1 = nv + sv;
+    p2 = nv - sv;
+  };
+  // CT 0: ring pairs wbase+4j+kq (K = 4 per product);
+  // CT 1: ring pairs wbase+32kb+8kq+jj, split (K = 32 per product)
+  T bp[2][CT ? 1 : 16][NCB];
+  Bf3 bq[2][CT ? 2 : 1][NCB];
+  if constexpr (CT == 0) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        double p1, p2;
+        bpair(wbase + 4 * j + kq, cb, p1, p2);
+        bp[0][j][cb] = (T)p1;
+        bp[1][j][cb] = (T)p2;
+      }
+  } else {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        float v1[8], v2[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          double p1, p2;
+          bpair(wbase + 32 * kb + 8 * kq + jj, cb, p1, p2);
+          v1[jj] = (float)p1;
+          v2[jj] = (float)p2;
+        }
+        bq[0][kb][cb] = split3(v1);
+        bq[1][kb][cb] = split3(v2);
+      }
   }
+  // slice index of local row ll (0..31) of this lane's ring pair
+  auto sidx = [&](int ll) { return CT ? ll * kRS + lane : lane * kLS + ll; };
 
   const double cth = a.cth[s], sth = a.sth[s];
   const double2 *rf = reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
@@ -780,8 +959,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
         lam1 = (cth * lam2) * bcast_d(cq, 4 * q) - lam1 * bcast_d(cq, 4 * q + 1);
         const double o1 = lam1;
         lam2 = (cth * lam1) * bcast_d(cq, 4 * q + 2) - lam2 * bcast_d(cq, 4 * q + 3);
-        mine[lane * kLS + 2 * q] = (T)o0;
-        mine[lane * kLS + 2 * q + 1] = (T)o1;
+        mine[sidx(2 * q)] = (T)o0;
+        mine[sidx(2 * q + 1)] = (T)o1;
       }
       l += 32;
     } else {
@@ -822,8 +1001,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mine[lane * kLS + 2 * j] = active ? (T)o0 : (T)0;
-        mine[lane * kLS + 2 * j + 1] = active ? (T)o1 : (T)0;
+        mine[sidx(2 * j)] = active ? (T)o0 : (T)0;
+        mine[sidx(2 * j + 1)] = active ? (T)o1 : (T)0;
         l += 2;
         if (q == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -850,14 +1029,28 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::zero();
     if (any) {
+      if constexpr (CT == 0) {
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const T A = mine[(4 * j + kq) * kLS + x + 2 * n];
+          for (int j = 0; j < 16; ++j) {
+            const T A = mine[(4 * j + kq) * kLS + x + 2 * n];
 #pragma unroll
-          for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::run(A, bp[x][j][cb], t[x][cb]);
-        }
+            for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::run(A, bp[x][j][cb], t[x][cb]);
+          }
+      } else {
+        // A: rows x+2n (l = lc+x+2n), ring pairs 32kb+8kq..+7
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb) {
+            float av[8];
+            slice_row8(reinterpret_cast<const float *>(mine), x + 2 * n, 32 * kb + 8 * kq, av);
+            const Bf3 A = split3(av);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma_bf3(A, bq[x][kb][cb], t[x][cb]);
+          }
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -869,10 +1062,11 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   }
 }
 
-template <typename T, int NCB>
+template <typename T, int NCB, int CT>
 __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
                                 T (&lds)[kMixWavesS][2][64 * kLS])
 {
+  static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
   constexpr int W = kMixWavesS;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
@@ -901,25 +1095,51 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
 
   // B: P[0] = N + sg*S and P[1] = N - sg*S (columns Q.re, Q.im, U.re, U.im)
-  T bp[2][16][NCB];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int rs = wbase + 4 * j + kq;
+  // of a ring pair rs
+  auto bpair = [&](int rs, int cb, double &p1, double &p2) {
     const bool ok = rs < nslots && a.mlim[rs < nslots ? rs : 0] >= m;
     const bool pr = ok && a.ispair[rs] != 0;
-#pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) {
-      const int col = cb * 16 + n;
-      double nv = 0., sv = 0.;
-      if (ok && col < ncol) {
-        const int comp = 2 * a.t0 + (col >> 1), part = col & 1;
-        nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
-        if (pr) sv = sg * ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
-      }
-      bp[0][j][cb] = (T)(nv + sv);
-      bp[1][j][cb] = (T)(nv - sv);
+    const int col = cb * 16 + n;
+    double nv = 0., sv = 0.;
+    if (ok && col < ncol) {
+      const int comp = 2 * a.t0 + (col >> 1), part = col & 1;
+      nv = ph[2 * phase_index(comp, m, rs, 0, nslots, a.nmb) + part];
+      if (pr) sv = sg * ph[2 * phase_index(comp, m, rs, 1, nslots, a.nmb) + part];
     }
+    p1 = nv + sv;
+    p2 = nv - sv;
+  };
+  // CT 0: ring pairs wbase+4j+kq; CT 1: ring pairs wbase+32kb+8kq+jj, split
+  T bp[2][CT ? 1 : 16][NCB];
+  Bf3 bq[2][CT ? 2 : 1][NCB];
+  if constexpr (CT == 0) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        double p1, p2;
+        bpair(wbase + 4 * j + kq, cb, p1, p2);
+        bp[0][j][cb] = (T)p1;
+        bp[1][j][cb] = (T)p2;
+      }
+  } else {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        float v1[8], v2[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          double p1, p2;
+          bpair(wbase + 32 * kb + 8 * kq + jj, cb, p1, p2);
+          v1[jj] = (float)p1;
+          v2[jj] = (float)p2;
+        }
+        bq[0][kb][cb] = split3(v1);
+        bq[1][kb][cb] = split3(v2);
+      }
   }
+  auto sidx = [&](int ll) { return CT ? ll * kRS + lane : lane * kLS + ll; };
   // column 4t+q of the lambda_x product takes P's column 4t+3-q with this sign
   // (a(E,B) += lw*P[x] + lx*rot(P[x^1]), libsharp/sharp_core_inc2.c:339-364)
   const T rsgn = ((n & 3) == 0 || (n & 3) == 3) ? (T)-1 : (T)1;
@@ -975,10 +1195,10 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
         const double rp1 = rec1p, rm1 = rec1m;
         rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
-        mw[lane * kLS + 2 * j] = (T)(rp0 + rm0);
-        mw[lane * kLS + 2 * j + 1] = (T)(rp1 + rm1);
-        mx[lane * kLS + 2 * j] = (T)(rm0 - rp0);
-        mx[lane * kLS + 2 * j + 1] = (T)(rm1 - rp1);
+        mw[sidx(2 * j)] = (T)(rp0 + rm0);
+        mw[sidx(2 * j + 1)] = (T)(rp1 + rm1);
+        mx[sidx(2 * j)] = (T)(rm0 - rp0);
+        mx[sidx(2 * j + 1)] = (T)(rm1 - rp1);
       }
       l += 32;
     } else {
@@ -1028,10 +1248,10 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mw[lane * kLS + 2 * j] = active ? (T)lw0 : (T)0;
-        mw[lane * kLS + 2 * j + 1] = active ? (T)lw1 : (T)0;
-        mx[lane * kLS + 2 * j] = active ? (T)lx0 : (T)0;
-        mx[lane * kLS + 2 * j + 1] = active ? (T)lx1 : (T)0;
+        mw[sidx(2 * j)] = active ? (T)lw0 : (T)0;
+        mw[sidx(2 * j + 1)] = active ? (T)lw1 : (T)0;
+        mx[sidx(2 * j)] = active ? (T)lx0 : (T)0;
+        mx[sidx(2 * j + 1)] = active ? (T)lx1 : (T)0;
         l += 2;
         if (h == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -1056,18 +1276,35 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) tw[x][cb] = tx[x][cb] = MM::zero();
     if (any) {
+      if constexpr (CT == 0) {
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int off = (4 * j + kq) * kLS + x + 2 * n;
-          const T Aw = mw[off], Ax = mx[off];
+          for (int j = 0; j < 16; ++j) {
+            const int off = (4 * j + kq) * kLS + x + 2 * n;
+            const T Aw = mw[off], Ax = mx[off];
 #pragma unroll
-          for (int cb = 0; cb < NCB; ++cb) {
-            tw[x][cb] = MM::run(Aw, bp[x][j][cb], tw[x][cb]);
-            tx[x][cb] = MM::run(Ax, bp[x ^ 1][j][cb], tx[x][cb]);
+            for (int cb = 0; cb < NCB; ++cb) {
+              tw[x][cb] = MM::run(Aw, bp[x][j][cb], tw[x][cb]);
+              tx[x][cb] = MM::run(Ax, bp[x ^ 1][j][cb], tx[x][cb]);
+            }
           }
-        }
+      } else {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb) {
+            float aw[8], ax[8];
+            slice_row8(reinterpret_cast<const float *>(mw), x + 2 * n, 32 * kb + 8 * kq, aw);
+            slice_row8(reinterpret_cast<const float *>(mx), x + 2 * n, 32 * kb + 8 * kq, ax);
+            const Bf3 Aw = split3(aw), Ax = split3(ax);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+              tw[x][cb] = mfma_bf3(Aw, bq[x][kb][cb], tw[x][cb]);
+              tx[x][cb] = mfma_bf3(Ax, bq[x ^ 1][kb][cb], tx[x][cb]);
+            }
+          }
+      }
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1097,27 +1334,29 @@ __global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
   if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin0_tile<NCB>(a, mb, sb);
 }
 
-template <int NCB, bool DERIV>
+template <int NCB, bool DERIV, int CT>
 __global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
 {
   int mb, sb;
-  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin_tile<NCB, DERIV>(a, mb, sb);
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin_tile<NCB, DERIV, CT>(a, mb, sb);
 }
 
-template <typename T, int NCB>
+template <typename T, int NCB, int CT>
 __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
 {
-  __shared__ T lds[kMixWaves0][64 * kLS];
+  __shared__ __align__(16) T lds[kMixWaves0][64 * kLS];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin0_tile<T, NCB>(a, m, g, lds);
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
+    manal_spin0_tile<T, NCB, CT>(a, m, g, lds);
 }
 
-template <typename T, int NCB>
+template <typename T, int NCB, int CT>
 __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
 {
-  __shared__ T lds[kMixWavesS][2][64 * kLS];
+  __shared__ __align__(16) T lds[kMixWavesS][2][64 * kLS];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g)) manal_spin_tile<T, NCB>(a, m, g, lds);
+  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
+    manal_spin_tile<T, NCB, CT>(a, m, g, lds);
 }
 
 // Persistent form: workgroup b works through the items of XCD b%8 (the work
@@ -1157,7 +1396,7 @@ __global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *jobs,
                                                           const BatchItem *items, const int *xofs)
 {
   for_items(items, xofs, [&](const BatchItem &it) {
-    msynth_spin_tile<NCB, DERIV>(job_args(jobs, it), it.major, it.minor);
+    msynth_spin_tile<NCB, DERIV, 0>(job_args(jobs, it), it.major, it.minor);
   });
 }
 
@@ -1168,7 +1407,7 @@ __global__ void __launch_bounds__(64 * kMixWaves0) kb_manal_spin0(const LegArgs 
 {
   __shared__ float lds[kMixWaves0][64 * kLS];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin0_tile<float, NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin0_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds);
     __syncthreads();   // the next item reuses the LDS slices
   });
 }
@@ -1180,7 +1419,7 @@ __global__ void __launch_bounds__(64 * kMixWavesS) kb_manal_spin(const LegArgs *
 {
   __shared__ float lds[kMixWavesS][2][64 * kLS];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin_tile<float, NCB>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds);
     __syncthreads();
   });
 }
@@ -1199,7 +1438,7 @@ static void check_launch(const char *what)
   }
 }
 
-void launch_msynth(const LegArgs &a, hipStream_t st)
+void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
 {
   const int nsb = (a.nslots + 63) / 64;
   const dim3 grid(xcd_grid(a.nmb, nsb));
@@ -1210,12 +1449,22 @@ void launch_msynth(const LegArgs &a, hipStream_t st)
   if (a.spin == 0) {
     if (two) hipLaunchKernelGGL(k_msynth_spin0<2>, grid, block, 0, st, a, nsb);
     else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
+  } else if (contr == kContrBf16x3) {
+    // spin 0 keeps the FP32 matrix cores: with 16 columns the contraction is
+    // too small a share for the splits to pay
+    if (a.deriv1) {
+      if (two) hipLaunchKernelGGL((k_msynth_spin<2, true, 1>), grid, block, 0, st, a, nsb);
+      else hipLaunchKernelGGL((k_msynth_spin<1, true, 1>), grid, block, 0, st, a, nsb);
+    } else {
+      if (two) hipLaunchKernelGGL((k_msynth_spin<2, false, 1>), grid, block, 0, st, a, nsb);
+      else hipLaunchKernelGGL((k_msynth_spin<1, false, 1>), grid, block, 0, st, a, nsb);
+    }
   } else if (a.deriv1) {
-    if (two) hipLaunchKernelGGL((k_msynth_spin<2, true>), grid, block, 0, st, a, nsb);
-    else hipLaunchKernelGGL((k_msynth_spin<1, true>), grid, block, 0, st, a, nsb);
+    if (two) hipLaunchKernelGGL((k_msynth_spin<2, true, 0>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_msynth_spin<1, true, 0>), grid, block, 0, st, a, nsb);
   } else {
-    if (two) hipLaunchKernelGGL((k_msynth_spin<2, false>), grid, block, 0, st, a, nsb);
-    else hipLaunchKernelGGL((k_msynth_spin<1, false>), grid, block, 0, st, a, nsb);
+    if (two) hipLaunchKernelGGL((k_msynth_spin<2, false, 0>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_msynth_spin<1, false, 0>), grid, block, 0, st, a, nsb);
   }
   check_launch("mixed synthesis");
 }
@@ -1253,22 +1502,31 @@ void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items
   check_launch("batched Legendre stage");
 }
 
-void launch_manal(const LegArgs &a, bool f64, hipStream_t st)
+void launch_manal(const LegArgs &a, int contr, hipStream_t st)
 {
+  const bool f64 = contr == kContrF64, bf = contr == kContrBf16x3;
   const dim3 grid(xcd_grid(a.mmax + 1, a.ngroups));
   const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
   if (a.ntl < 1 || cols > (f64 ? 16 : 32)) SHP_FAIL("bad transform group");
   const bool two = cols > 16;
   if (f64) {
-    if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_manal_spin<double, 1>), grid, block, 0, st, a);
+    if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1, 0>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin<double, 1, 0>), grid, block, 0, st, a);
+  } else if (bf) {
+    if (a.spin == 0) {
+      if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2, 1>), grid, block, 0, st, a);
+      else hipLaunchKernelGGL((k_manal_spin0<float, 1, 1>), grid, block, 0, st, a);
+    } else {
+      if (two) hipLaunchKernelGGL((k_manal_spin<float, 2, 1>), grid, block, 0, st, a);
+      else hipLaunchKernelGGL((k_manal_spin<float, 1, 1>), grid, block, 0, st, a);
+    }
   } else if (a.spin == 0) {
-    if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_manal_spin0<float, 1>), grid, block, 0, st, a);
+    if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2, 0>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin0<float, 1, 0>), grid, block, 0, st, a);
   } else {
-    if (two) hipLaunchKernelGGL((k_manal_spin<float, 2>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_manal_spin<float, 1>), grid, block, 0, st, a);
+    if (two) hipLaunchKernelGGL((k_manal_spin<float, 2, 0>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_manal_spin<float, 1, 0>), grid, block, 0, st, a);
   }
   check_launch(f64 ? "FP64 matrix-core analysis" : "mixed analysis");
 }

@@ -268,7 +268,11 @@ __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
   double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
   double f = a.norm_l[l];
   if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
-  a.almtmp[(int64_t)tri_ofs(m, a.lmax) * nc + e] = make_double2(re * f, im * f);
+  const int64_t o = (int64_t)tri_ofs(m, a.lmax) * nc + e;
+  if (a.tmp_f32)
+    reinterpret_cast<float2 *>(a.almtmp)[o] = make_float2((float)(re * f), (float)(im * f));
+  else
+    a.almtmp[o] = make_double2(re * f, im * f);
 }
 
 // components [c0, c0+npc); the values come from almtmp, or (part != nullptr)

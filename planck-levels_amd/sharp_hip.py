@@ -30,7 +30,7 @@ LIB_PATH = os.environ.get("SHARP_HIP_LIBRARY") or os.path.join(HERE, "libsharp_h
 # job types and flags (libsharp/sharp_lowlevel.h:162-191, include/sharp_hip.h)
 MAP2ALM, ALM2MAP, Yt, WY, ALM2MAP_DERIV1 = 0, 1, 2, 3, 4
 DP, ADD, REAL_HARMONICS, NO_FFT = 1 << 4, 1 << 5, 1 << 6, 1 << 7
-MIXED_FP32 = 1 << 24
+MIXED_FP32, MIXED_BF16X3 = 1 << 24, 1 << 25
 PACKED = 1
 
 

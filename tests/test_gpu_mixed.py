@@ -1,9 +1,10 @@
-"""Mixed-precision form (SHARP_HIP_MIXED_FP32): FP64 recurrence with exponent
-scaling, FP32 matrix-core contraction.  Its outputs are compared with the FP64
-oracle (small cases) or the FP64 GPU path (larger cases, itself pinned to the
-oracle and the reference build at 1e-12).
+"""Mixed-precision forms: FP64 recurrence with exponent scaling, contraction
+on the matrix cores -- FP32 MFMA (SHARP_HIP_MIXED_FP32) or BF16 MFMA on exact
+three-piece BF16 splits of the FP32 operands (SHARP_HIP_MIXED_BF16X3).  The
+outputs are compared with the FP64 oracle (small cases) or the FP64 GPU path
+(larger cases, itself pinned to the oracle and the reference build at 1e-12).
 
-Tolerance: 1e-6 relative L2 per output array, the bound the mixed form is
+Tolerance: 1e-6 relative L2 per output array, the bound the mixed forms are
 specified to meet.  Every case also has to be far from FP64-exact (>1e-12),
 which shows that the MFMA path, not the FP64 one, produced it.
 """
@@ -17,12 +18,18 @@ from test_gpu_parity import _general_geom
 
 pytestmark = pytest.mark.gpu
 
-MIXED = 1 << 24     # SHARP_HIP_MIXED_FP32 (include/sharp_hip.h)
+MIXED_FP32 = 1 << 24     # SHARP_HIP_MIXED_FP32 (include/sharp_hip.h)
+MIXED_BF16X3 = 1 << 25   # SHARP_HIP_MIXED_BF16X3
 TOLMIX = 1e-6
 
 
-def compare_mixed(product, checker, *args, flags=O.DP, tol=TOLMIX, **kw):
-    pa, pm = run_case(product, *args, flags | MIXED, **kw)
+@pytest.fixture(params=[MIXED_FP32, MIXED_BF16X3], ids=["fp32", "bf16x3"])
+def form(request):
+    return request.param
+
+
+def compare_mixed(product, checker, *args, flags=O.DP, tol=TOLMIX, form=MIXED_FP32, **kw):
+    pa, pm = run_case(product, *args, flags | form, **kw)
     ca, cm = run_case(checker, *args, flags, **kw)
     errs = [rel_l2(x, y) for x, y in zip(pa + pm, ca + cm)]
     print("mixed rel err", args[2:], max(errs))
@@ -33,55 +40,55 @@ def compare_mixed(product, checker, *args, flags=O.DP, tol=TOLMIX, **kw):
 @pytest.mark.parametrize("spin", [0, 2])
 @pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
 @pytest.mark.parametrize("ntrans", [1, 3, 8])
-def test_healpix(product, oracle, gpu, spin, jobtype, ntrans):
-    compare_mixed(product, oracle, healpix(16), tri(32), jobtype, spin, ntrans)
+def test_healpix(product, oracle, gpu, spin, jobtype, ntrans, form):
+    compare_mixed(product, oracle, healpix(16), tri(32), jobtype, spin, ntrans, form=form)
 
 
 @pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM, O.Yt, O.WY])
 @pytest.mark.parametrize("spin", [0, 1, 2, 3])
-def test_gauss_jobtypes(product, oracle, gpu, jobtype, spin):
-    compare_mixed(product, oracle, iso("gauss", 25, 49), tri(24), jobtype, spin, 2)
+def test_gauss_jobtypes(product, oracle, gpu, jobtype, spin, form):
+    compare_mixed(product, oracle, iso("gauss", 25, 49), tri(24), jobtype, spin, 2, form=form)
 
 
 @pytest.mark.parametrize("spin,ntrans", [(0, 16), (0, 17), (2, 9), (1, 11)])
-def test_several_launches(product, oracle, gpu, spin, ntrans):
+def test_several_launches(product, oracle, gpu, spin, ntrans, form):
     """More transforms than one launch holds (16 for spin 0, 8 for spin s)."""
     for jt in (O.ALM2MAP, O.MAP2ALM):
-        compare_mixed(product, oracle, iso("gauss", 17, 33), tri(16), jt, spin, ntrans)
+        compare_mixed(product, oracle, iso("gauss", 17, 33), tri(16), jt, spin, ntrans, form=form)
 
 
-def test_deriv1(product, oracle, gpu):
-    compare_mixed(product, oracle, healpix(8), tri(20), O.DERIV1, 1, 2)
+def test_deriv1(product, oracle, gpu, form):
+    compare_mixed(product, oracle, healpix(8), tri(20), O.DERIV1, 1, 2, form=form)
 
 
 @pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
-def test_flags_and_layouts(product, oracle, gpu, jobtype):
+def test_flags_and_layouts(product, oracle, gpu, jobtype, form):
     compare_mixed(product, oracle, healpix(8), tri(18), jobtype, 2, 2,
-                  flags=O.DP | O.ADD | O.REAL_HARMONICS)
-    compare_mixed(product, oracle, healpix(8), rect(20, 12), jobtype, 0, 3)
-    compare_mixed(product, oracle, healpix(8), tri(20, 7), jobtype, 2, 1)
+                  flags=O.DP | O.ADD | O.REAL_HARMONICS, form=form)
+    compare_mixed(product, oracle, healpix(8), rect(20, 12), jobtype, 0, 3, form=form)
+    compare_mixed(product, oracle, healpix(8), tri(20, 7), jobtype, 2, 1, form=form)
     compare_mixed(product, oracle, healpix(8), tri(16), jobtype, 0, 2, flags=0,
-                  tol=2e-6)
+                  tol=2e-6, form=form)
 
 
 @pytest.mark.parametrize("spin", [0, 1, 2])
 @pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
-def test_general_geometry(product, oracle, gpu, spin, jobtype):
-    compare_mixed(product, oracle, _general_geom, tri(10), jobtype, spin, 2)
+def test_general_geometry(product, oracle, gpu, spin, jobtype, form):
+    compare_mixed(product, oracle, _general_geom, tri(10), jobtype, spin, 2, form=form)
 
 
-def test_high_spin(product, oracle, gpu):
+def test_high_spin(product, oracle, gpu, form):
     """spin 30: rows l < spin carry no lambda and stay zero."""
     for jt in (O.ALM2MAP, O.MAP2ALM):
-        compare_mixed(product, oracle, iso("gauss", 64, 127), tri(63), jt, 30, 1)
+        compare_mixed(product, oracle, iso("gauss", 64, 127), tri(63), jt, 30, 1, form=form)
 
 
 @pytest.mark.parametrize("spin", [0, 2])
-def test_larger_vs_fp64(product, gpu, spin):
+def test_larger_vs_fp64(product, gpu, spin, form):
     """HEALPix nside 128, lmax 256, 8 transforms, against the FP64 GPU path;
     both directions, and far from FP64-exact."""
     for jt in (O.ALM2MAP, O.MAP2ALM):
-        pa, pm = run_case(product, healpix(128), tri(256), jt, spin, 8, O.DP | MIXED)
+        pa, pm = run_case(product, healpix(128), tri(256), jt, spin, 8, O.DP | form)
         ca, cm = run_case(product, healpix(128), tri(256), jt, spin, 8, O.DP)
         outs = zip(pm, cm) if jt == O.ALM2MAP else zip(pa, ca)
         errs = [rel_l2(x, y) for x, y in outs]
