@@ -155,25 +155,46 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a)
       if (!done) {
         lam1 *= cf;
         lam2 *= cf;
-        // hot loop: 3 FP64 ops of recurrence + 4*TG of accumulation per l pair/2
+        // hot loop: 3 FP64 ops of recurrence + 4*TG of accumulation per l
+        // pair.  The next pair's coefficients and a_lm (scalar loads) are
+        // requested right after the first use of the current ones (which
+        // waits for every scalar load in flight), so they have a whole pair
+        // to arrive; rows up to lmax+2 are readable (table padding, next m
+        // or the slack rows of the canonical array).
+        double2 r0 = rf[l], r1 = rf[l + 1];
+        double2 x0[TG], x1[TG];
+#pragma unroll
+        for (int t = 0; t < TG; ++t) {
+          x0[t] = al[(int64_t)l * nc + t];
+          x1[t] = al[(int64_t)(l + 1) * nc + t];
+        }
         while (l < lmax) {
-          const double2 r0 = rf[l], r1 = rf[l + 1];
           lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+          __builtin_amdgcn_sched_barrier(0);
+          const double2 nr0 = rf[l + 2], nr1 = rf[l + 3];
+          double2 nx0[TG], nx1[TG];
 #pragma unroll
           for (int t = 0; t < TG; ++t) {
-            const double2 x0 = al[(int64_t)l * nc + t];
-            const double2 x1 = al[(int64_t)(l + 1) * nc + t];
-            p1r[t] = fma(lam2, x0.x, p1r[t]); p1i[t] = fma(lam2, x0.y, p1i[t]);
-            p2r[t] = fma(lam1, x1.x, p2r[t]); p2i[t] = fma(lam1, x1.y, p2i[t]);
+            nx0[t] = al[(int64_t)(l + 2) * nc + t];
+            nx1[t] = al[(int64_t)(l + 3) * nc + t];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            p1r[t] = fma(lam2, x0[t].x, p1r[t]); p1i[t] = fma(lam2, x0[t].y, p1i[t]);
+            p2r[t] = fma(lam1, x1[t].x, p2r[t]); p2i[t] = fma(lam1, x1[t].y, p2i[t]);
           }
           lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+          r0 = nr0;
+          r1 = nr1;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) { x0[t] = nx0[t]; x1[t] = nx1[t]; }
           l += 2;
         }
         if (l == lmax) {
 #pragma unroll
           for (int t = 0; t < TG; ++t) {
-            const double2 x = al[(int64_t)l * nc + t];
-            p1r[t] += lam2 * x.x; p1i[t] += lam2 * x.y;
+            p1r[t] += lam2 * x0[t].x; p1i[t] += lam2 * x0[t].y;
           }
         }
       }
@@ -283,20 +304,45 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a)
       }
       if (!done) {
         rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+        // hot loop, the next pair's loads requested after the first use of
+        // the current ones (as in the spin-0 kernel); rows up to lmax+2 and
+        // coefficients up to lmax+3 are readable
+        Fx f1 = fx(l + 1), f2 = fx(l + 2);
+        double2 g0[TG], c0[TG], g1[TG], c1[TG];
+#pragma unroll
+        for (int t = 0; t < TG; ++t) {
+          g0[t] = ag(l, t); c0[t] = ac(l, t);
+          g1[t] = ag(l + 1, t); c1[t] = ac(l + 1, t);
+        }
         while (l < lmax) {
-          rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+          rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
+          __builtin_amdgcn_sched_barrier(0);
+          const Fx nf1 = fx(l + 3), nf2 = fx(l + 4);
+          double2 ng0[TG], nc0[TG], ng1[TG], nc1[TG];
 #pragma unroll
           for (int t = 0; t < TG; ++t) {
-            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, ag(l, t), ac(l, t));
-            saddstep<DERIV>(p2[t], p1[t], rec1p, rec1m, ag(l + 1, t), ac(l + 1, t));
+            ng0[t] = ag(l + 2, t); nc0[t] = ac(l + 2, t);
+            ng1[t] = ag(l + 3, t); nc1[t] = ac(l + 3, t);
           }
-          rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, g0[t], c0[t]);
+            saddstep<DERIV>(p2[t], p1[t], rec1p, rec1m, g1[t], c1[t]);
+          }
+          rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
+          f1 = nf1;
+          f2 = nf2;
+#pragma unroll
+          for (int t = 0; t < TG; ++t) {
+            g0[t] = ng0[t]; c0[t] = nc0[t]; g1[t] = ng1[t]; c1[t] = nc1[t];
+          }
           l += 2;
         }
         if (l == lmax) {
 #pragma unroll
           for (int t = 0; t < TG; ++t)
-            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, ag(l, t), ac(l, t));
+            saddstep<DERIV>(p1[t], p2[t], rec2p, rec2m, g0[t], c0[t]);
         }
       }
     }
