@@ -5,7 +5,8 @@
 // Job normalisation and checks follow libsharp/sharp.c:238-251,762-880 and
 // libsharp/sharp_ylmgen_c.c:44-52; the recurrence tables are built exactly as
 // libsharp/sharp_ylmgen_c.c:44-232 builds them (same operations in the same
-// order), so both implementations start from bit-identical coefficients.
+// order), so both implementations start from bit-identical coefficients; the
+// spin-s f1 is stored premultiplied by f0 (one product more, see rec_step).
 //
 // Pipeline per call (one HIP stream, no host synchronisation inside):
 //   synthesis: alm_gather -> Legendre synthesis (per transform group)
@@ -504,7 +505,8 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
         const double lt = 2 * l + 1, l1 = l + 1;
         double *e = row + 3 * (l + 1 - m);
         e[0] = l1 * lt * t;
-        e[1] = m * s * inv[l] * inv[l + 1];
+        // stored as f0*f1: the kernels form (cth -+ f1) f0 as one FMA
+        e[1] = e[0] * (m * s * inv[l] * inv[l + 1]);
         t = flm2[l + m] * flm2[l - m] * flm2[l + s] * flm2[l - s];
         e[2] = t * l1 * inv[l];
       }

@@ -100,13 +100,17 @@ __device__ inline void spin_init(double cth, double sth, int s, const SpinStart 
   normalize(rec2p, scalep, kFTol);
 }
 
-struct Fx { double f0, f1, f2; };
+// spin-s recurrence coefficients of one l: libsharp's fx[l] = {f0, f1, f2}
+// (libsharp/sharp_ylmgen_c.c:174-183) with f1 stored premultiplied, g = f0*f1
+struct Fx { double f0, g, f2; };
 
+// x_{l+1} = (cth -+ f1) f0 y_l - f2 x_{l-1} (libsharp/sharp_core_inc.c:
+// 209-221), the factor (cth -+ f1) f0 = cth f0 -+ g as one FMA
 __device__ inline void rec_step(double &rxp, double &rxm, double ryp, double rym,
                                 double cth, const Fx &f)
 {
-  rxp = ((cth - f.f1) * (f.f0 * ryp)) - f.f2 * rxp;
-  rxm = ((cth + f.f1) * (f.f0 * rym)) - f.f2 * rxm;
+  rxp = fma(fma(cth, f.f0, -f.g), ryp, -(f.f2 * rxp));
+  rxm = fma(fma(cth, f.f0, f.g), rym, -(f.f2 * rxm));
 }
 
 __device__ inline bool rescale1(double &l1, double &l2, int &scale)

@@ -726,7 +726,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
             // The first use of this block's coefficients waits for every
             // scalar load in flight; the next block's are requested right
             // after it, so they have a whole block to arrive.
-            const double c0m = cth - fc[0].f1, c0p = cth + fc[0].f1;
+            const double c0m = fma(cth, fc[0].f0, -fc[0].g), c0p = fma(cth, fc[0].f0, fc[0].g);
             __builtin_amdgcn_sched_barrier(0);
             // lb + 8 <= lmax + kTabPad in this phase
 #pragma unroll
@@ -734,8 +734,8 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
             __builtin_amdgcn_sched_barrier(0);
             {
               const double rp0 = rec2p, rm0 = rec2m;
-              rec1p = (c0m * (fc[0].f0 * rec2p)) - fc[0].f2 * rec1p;
-              rec1m = (c0p * (fc[0].f0 * rec2m)) - fc[0].f2 * rec1m;
+              rec1p = fma(c0m, rec2p, -(fc[0].f2 * rec1p));
+              rec1m = fma(c0p, rec2m, -(fc[0].f2 * rec1m));
               rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[1]);
               v1[0] = (float)(rp0 + rm0);
               v2[0] = (float)(rm0 - rp0);
