@@ -436,6 +436,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM counters)")
+    ap.add_argument("--streams", type=int, default=1, choices=[1, 2],
+                    help="streams for the T and Q/U transforms of a step")
     ap.add_argument("--host-path", action="store_true",
                     help="also time sharp_execute on host arrays (H2D/D2H copies "
                          "included; reported beside value, never as value)")
@@ -474,6 +476,10 @@ def main():
     EB = [x for pair in zip(E, B) for x in pair]
     zeros = lambda: torch.zeros(npix, dtype=torch.float64, device=dev)  # noqa: E731
     stream = torch.cuda.current_stream()
+    # default-priority side streams, ordered after the input generation
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for x in streams:
+        x.wait_stream(stream)
 
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).normal_()
@@ -494,11 +500,16 @@ def main():
         out_t = [torch.zeros_like(T[0]) for _ in range(batch)]
         out_eb = [torch.zeros_like(T[0]) for _ in range(2 * batch)]
 
+        # T (spin 0) and Q/U (spin 2) are independent: with --streams 2 they
+        # are enqueued on two streams (the library keeps work buffers per
+        # stream), so the kernels of one overlap those of the other
+        st_t, st_qu = (stream, stream) if args.streams == 1 else (streams[0], streams[1])
+
         def step():
-            S.execute_device(S.ALM2MAP, 0, T, map_t, geom, ainfo, batch, flags, stream)
-            S.execute_device(S.ALM2MAP, 2, EB, map_qu, geom, ainfo, batch, flags, stream)
-            S.execute_device(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags, stream)
-            S.execute_device(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags, stream)
+            S.execute_device(S.ALM2MAP, 0, T, map_t, geom, ainfo, batch, flags, st_t)
+            S.execute_device(S.ALM2MAP, 2, EB, map_qu, geom, ainfo, batch, flags, st_qu)
+            S.execute_device(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags, st_t)
+            S.execute_device(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags, st_qu)
 
         log("%s: warmup x%d, timed x%d" % (form, args.warmup, args.steps))
         dt, stats = run_form(S, torch, step, args.warmup, args.steps)
@@ -576,7 +587,8 @@ def main():
                 "EE=0.1TT, BB=0.01TT, TE corr 0.5",
         "config": {"workload": "HEALPix Nside %d, lmax %d, TQU, batch %d, "
                                "alm2map+map2alm" % (nside, lmax, batch),
-                   "nside": nside, "lmax": lmax, "batch": batch, "form": args.form},
+                   "nside": nside, "lmax": lmax, "batch": batch, "form": args.form,
+                   "streams": args.streams},
         "roofline": main_form["roofline"],
         "cpu_baseline": cpu,
     }

@@ -8,8 +8,10 @@
  * same meaning and count as in sharp_execute (libsharp/sharp_lowlevel.h:
  * 196-224); `stream` is a hipStream_t (NULL: the null stream of the current
  * device, as in HIP).  The call enqueues the work and returns; the caller
- * synchronises the stream before reading the outputs.  Argument errors
- * behave as in sharp_execute (message, then exit(1)).
+ * synchronises the stream before reading the outputs.  Each stream has its
+ * own library work buffers (kept until sharp_hip_release), so calls on
+ * different streams may run concurrently.  Argument errors behave as in
+ * sharp_execute (message, then exit(1)).
  */
 #ifndef SHARP_HIP_H
 #define SHARP_HIP_H

@@ -214,7 +214,19 @@ struct Ctx {
   std::vector<std::unique_ptr<YlmPlan>> ylms;
   std::vector<std::unique_ptr<NormPlan>> norms;
   std::vector<std::unique_ptr<AlmPlan>> alms;
-  DevBuf phase, almtmp, part, stage_alm, stage_map, resid, lfac;
+  // work buffers per launch stream: calls on different streams do not share
+  // them and may run concurrently
+  struct StreamBufs {
+    DevBuf phase, almtmp, part, resid, lfac;
+  };
+  std::map<hipStream_t, std::unique_ptr<StreamBufs>> sbufs;
+  StreamBufs &bufs(hipStream_t st)
+  {
+    auto &p = sbufs[st];
+    if (!p) p = std::make_unique<StreamBufs>();
+    return *p;
+  }
+  DevBuf stage_alm, stage_map;
   // pinned staging of the per-l factors; lfac_ev marks the end of the last
   // copy out of it
   double *lfac_host = nullptr;
@@ -736,7 +748,7 @@ static double *upload_lfac(Ctx &c, const Job &j, hipStream_t st)
     c.lfac_cap = bytes;
   }
   for (size_t l = 0; l < n.size(); ++l) c.lfac_host[l] = n[l] * j.lfac[l];
-  double *d = static_cast<double *>(c.lfac.get(bytes));
+  double *d = static_cast<double *>(c.bufs(st).lfac.get(bytes));
   HIP_CHECK(hipMemcpyAsync(d, c.lfac_host, bytes, hipMemcpyHostToDevice, st));
   HIP_CHECK(hipEventRecord(c.lfac_ev, st));
   return d;
@@ -979,14 +991,15 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
                        unsigned long long *opcnt)
 {
   GeomPlan &gp = get_geom(c, j.g);
-  double2 *phase = static_cast<double2 *>(c.phase.get(phase_elems(j, gp) * sizeof(double2)));
-  double2 *almtmp = static_cast<double2 *>(c.almtmp.get(almtmp_elems(j) * sizeof(double2)));
+  Ctx::StreamBufs &sb = c.bufs(st);
+  double2 *phase = static_cast<double2 *>(sb.phase.get(phase_elems(j, gp) * sizeof(double2)));
+  double2 *almtmp = static_cast<double2 *>(sb.almtmp.get(almtmp_elems(j) * sizeof(double2)));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
 
   const bool mixed = is_mixed(j.flags);
   if (j.type != SHARP_ALM2MAP) {
     stage_rings_to_phase(c, s, st);
-    stage_analysis(c, s, c.part, st);
+    stage_analysis(c, s, sb.part, st);
     return;
   }
   stage_gather(c, s, st);
@@ -1440,7 +1453,7 @@ void sharp_hip_map2alm_iter_device(int spin, void *alm, void *map,
   const int64_t n = rhi - rlo + 1;
   const int ncomp = ja.ntrans * ja.nmaps;
   const size_t esz = (flags & SHARP_DP) ? sizeof(double) : sizeof(float);
-  char *rb = static_cast<char *>(c.resid.get((size_t)n * esz * ncomp));
+  char *rb = static_cast<char *>(c.bufs(st).resid.get((size_t)n * esz * ncomp));
   std::vector<void *> rmap(ncomp);
   PtrTab src{}, dst{};
   for (int i = 0; i < ncomp; ++i) {
@@ -1603,13 +1616,13 @@ void sharp_hip_release(void)
   c.ylms.clear();
   c.norms.clear();
   c.alms.clear();
-  c.phase.release();
-  c.almtmp.release();
-  c.part.release();
+  for (auto &kv : c.sbufs) {
+    Ctx::StreamBufs &b = *kv.second;
+    for (DevBuf *d : {&b.phase, &b.almtmp, &b.part, &b.resid, &b.lfac}) d->release();
+  }
+  c.sbufs.clear();
   c.stage_alm.release();
   c.stage_map.release();
-  c.resid.release();
-  c.lfac.release();
 }
 
 void sharp_hip_profile(int enable)
