@@ -850,7 +850,7 @@ __device__ inline void write_tiles(T *slice, const typename Mfma<T>::acc (&t)[2]
 // matrix cores on exact three-piece splits of FP32 operands
 template <typename T, int NCB, int CT>
 __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
-                                 T (&lds)[kMixWaves0][64 * kLS])
+                                 T (&lds)[kMixWaves0][64 * kLS], double (&cbuf)[2][64])
 {
   static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
@@ -939,17 +939,24 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww];
-  // coefficients rf[lc .. lc+31] of a chunk as 64 doubles, one per lane,
-  // loaded a chunk ahead and broadcast with readlane in the unscaled chunks
+  // coefficients rf[lc .. lc+31] of a chunk (64 doubles) in LDS, shared by
+  // the workgroup's wavefronts (one m) and staged one chunk ahead; the
+  // unscaled chunks read them with wave-uniform addresses
   const double *rfd = reinterpret_cast<const double *>(rf);
   const int rdmax = 2 * lpad + 1;
-  double cq = rfd[min(2 * m + lane, rdmax)];
+  auto stage_coef = [&](int lc, double *dst) {
+    if (threadIdx.x < 64) dst[threadIdx.x] = rfd[min(2 * lc + (int)threadIdx.x, rdmax)];
+  };
+  stage_coef(m, cbuf[0]);
+  __syncthreads();
   double2 rc[8];
 
   const int nchunks = (lmax - m + 32) / 32;
   for (int c = 0; c < nchunks; ++c) {
     const int lc = m + 32 * c;
-    const double cqn = rfd[min(2 * (lc + 32) + lane, rdmax)];
+    // the other buffer was last read in chunk c-1, before its barriers
+    stage_coef(lc + 32, cbuf[(c + 1) & 1]);
+    const double *cq = cbuf[c & 1];
     bool any = false;
     if (mode == 2 && lc + 31 <= lmax) {
       // unscaled and whole: no checks; lambda of every lane stays in range
@@ -957,9 +964,9 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const double o0 = lam2;
-        lam1 = (cth * lam2) * bcast_d(cq, 4 * q) - lam1 * bcast_d(cq, 4 * q + 1);
+        lam1 = (cth * lam2) * cq[4 * q] - lam1 * cq[4 * q + 1];
         const double o1 = lam1;
-        lam2 = (cth * lam1) * bcast_d(cq, 4 * q + 2) - lam2 * bcast_d(cq, 4 * q + 3);
+        lam2 = (cth * lam1) * cq[4 * q + 2] - lam2 * cq[4 * q + 3];
         mine[sidx(2 * q)] = (T)o0;
         mine[sidx(2 * q + 1)] = (T)o1;
       }
@@ -1017,7 +1024,6 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
     }
     mode = __builtin_amdgcn_readfirstlane(mode);
     }
-    cq = cqn;
     // the slice is read back by the same wavefront: LDS operations of one
     // wavefront complete in order, the fence keeps the compiler from moving
     // the loads above the stores
@@ -1065,7 +1071,7 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
 
 template <typename T, int NCB, int CT>
 __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
-                                T (&lds)[kMixWavesS][2][64 * kLS])
+                                T (&lds)[kMixWavesS][2][64 * kLS], double (&cbuf)[2][96])
 {
   static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
@@ -1165,33 +1171,31 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
-  // coefficients fx(lc+1 .. lc+32) of a chunk as 96 doubles, two per lane
-  // in lanes 0..47, loaded a chunk ahead and broadcast with readlane in the
-  // unscaled chunks
+  // coefficients fx(lc+1 .. lc+32) of a chunk (96 doubles) in LDS, shared
+  // by the workgroup's wavefronts (one m) and staged one chunk ahead; the
+  // unscaled chunks read them with wave-uniform addresses
   const int fdmax = 3 * lpad + 2;
-  auto chunk_coef = [&](int lc, double &c0, double &c1) {
-    const int b = 3 * (lc + 1) + 2 * min(lane, 47);
-    c0 = fxb[min(b, fdmax)];
-    c1 = fxb[min(b + 1, fdmax)];
+  auto stage_coef = [&](int lc, double *dst) {
+    if (threadIdx.x < 96) dst[threadIdx.x] = fxb[min(3 * (lc + 1) + (int)threadIdx.x, fdmax)];
   };
-  double cqa, cqb;
-  chunk_coef(st.mhi, cqa, cqb);
-  auto cf_at = [&](int i) { return bcast_d((i & 1) ? cqb : cqa, i >> 1); };
+  stage_coef(st.mhi, cbuf[0]);
+  __syncthreads();
   Fx fc[4];
 
   const int nchunks = (lmax - st.mhi + 32) / 32;
   for (int c = 0; c < nchunks; ++c) {
     const int lc = st.mhi + 32 * c;
-    double cqna, cqnb;
-    chunk_coef(lc + 32, cqna, cqnb);
+    // the other buffer was last read in chunk c-1, before its barriers
+    stage_coef(lc + 32, cbuf[(c + 1) & 1]);
+    const double *cq = cbuf[c & 1];
     bool any = false;
     if (mode == 2 && lc + 31 <= lmax) {
       // unscaled and whole: no checks; lambda of every lane stays in range
       any = true;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const Fx f1{cf_at(6 * j), cf_at(6 * j + 1), cf_at(6 * j + 2)};
-        const Fx f2{cf_at(6 * j + 3), cf_at(6 * j + 4), cf_at(6 * j + 5)};
+        const Fx f1{cq[6 * j], cq[6 * j + 1], cq[6 * j + 2]};
+        const Fx f2{cq[6 * j + 3], cq[6 * j + 4], cq[6 * j + 5]};
         const double rp0 = rec2p, rm0 = rec2m;
         rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
         const double rp1 = rec1p, rm1 = rec1m;
@@ -1266,8 +1270,6 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     }
     mode = __builtin_amdgcn_readfirstlane(mode);
     }
-    cqa = cqna;
-    cqb = cqnb;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1346,18 +1348,20 @@ template <typename T, int NCB, int CT>
 __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
 {
   __shared__ __align__(16) T lds[kMixWaves0][64 * kLS];
+  __shared__ double cbuf[2][64];
   int m, g;
   if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin0_tile<T, NCB, CT>(a, m, g, lds);
+    manal_spin0_tile<T, NCB, CT>(a, m, g, lds, cbuf);
 }
 
 template <typename T, int NCB, int CT>
 __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
 {
   __shared__ __align__(16) T lds[kMixWavesS][2][64 * kLS];
+  __shared__ double cbuf[2][96];
   int m, g;
   if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin_tile<T, NCB, CT>(a, m, g, lds);
+    manal_spin_tile<T, NCB, CT>(a, m, g, lds, cbuf);
 }
 
 // Persistent form: workgroup b works through the items of XCD b%8 (the work
@@ -1407,8 +1411,9 @@ __global__ void __launch_bounds__(64 * kMixWaves0) kb_manal_spin0(const LegArgs 
                                                                   const int *xofs)
 {
   __shared__ float lds[kMixWaves0][64 * kLS];
+  __shared__ double cbuf[2][64];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin0_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin0_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
     __syncthreads();   // the next item reuses the LDS slices
   });
 }
@@ -1419,8 +1424,9 @@ __global__ void __launch_bounds__(64 * kMixWavesS) kb_manal_spin(const LegArgs *
                                                                  const int *xofs)
 {
   __shared__ float lds[kMixWavesS][2][64 * kLS];
+  __shared__ double cbuf[2][96];
   for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds);
+    manal_spin_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
     __syncthreads();
   });
 }
