@@ -101,13 +101,13 @@ void launch_phase2ring_dft(const RingArgs &a, const int *list, int nlist, hipStr
 void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStream_t st);
 
 // Per-ring FFT plan.  rad[] lists the Stockham radices of N (smooth lengths)
-// or of the Bluestein length M.
+// or of the Bluestein length M (8, then 8s and one 4 or 2, then a last 8).
 struct RingFftPlan {
   int ring;        // slot*2 + hemi
   int n;           // pixels
   int N;           // complex FFT length: n/2 (even n) or n (odd n)
   int bluestein;   // 1: N has a prime factor > 7
-  int M;           // Bluestein convolution length (7-smooth, >= 2N-1)
+  int M;           // Bluestein convolution length (power of two, >= 2N-1, >= 64)
   int nrad;
   int8_t rad[16];
   int64_t boff;    // Bluestein: offset of the plan's block in the table

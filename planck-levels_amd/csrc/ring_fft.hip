@@ -19,9 +19,12 @@
 // 5 and 7, in place in one LDS buffer (every pass reads all its inputs into
 // registers, barriers, then writes).  A length with another prime factor is
 // done by Bluestein's algorithm: a circular convolution of length
-// M >= 2N-1 (7-smooth) with the chirp e^{-i pi j^2 / N}; the chirp and the
-// FFT of the convolution kernel are precomputed once per (N, M) into a table
-// in HBM.  Twiddles of a pass come from one sincos per butterfly; the
+// M >= 2N-1 (a power of two) with the chirp e^{-i pi j^2 / N}; the chirp and
+// the FFT of the convolution kernel are precomputed once per (N, M) into a
+// table in HBM, and the pointwise products ride on the first and last
+// passes of the two length-M transforms.  The transform helpers are forced
+// inline: as device calls they cost registers, spills and a fifth of the
+// kernels' time.  Twiddles of a pass come from one sincos per butterfly; the
 // azimuthal rotations e^{+-i m phi0} from one sincos per group of bins,
 // advanced by complex multiplication.
 //
@@ -184,8 +187,21 @@ template <> struct Dft<5> : DftOdd<5, Tw5> {};
 template <> struct Dft<7> : DftOdd<7, Tw7> {};
 
 // --- one Stockham pass of radix R over buf[0..N) -----------------------------
-template <int NT, int E, int R>
-__device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
+// With FIRST the pass reads its inputs through `pre` (pre(buf, i) instead of
+// buf[i]); with LAST it passes its outputs through `post` (post(i, v)
+// updates v and returns whether buf[i] is written).  The Bluestein
+// convolution folds its pointwise products into them.
+struct PlainLoad {
+  __device__ double2 operator()(const double2 *b, int i) const { return b[i]; }
+};
+struct PlainStore {
+  __device__ bool operator()(int, double2 &) const { return true; }
+};
+
+template <int NT, int E, int R, bool FIRST = false, bool LAST = false,
+          typename Pre = PlainLoad, typename Post = PlainStore>
+__device__ __forceinline__ void stockham_pass(double2 *buf, int N, int Ns, int sign,
+                                     const Pre &pre = Pre(), const Post &post = Post())
 {
   constexpr int BPT = (E + R - 1) / R;
   const int tid = threadIdx.x;
@@ -196,19 +212,25 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
     const int j = tid + i * NT;
     if (j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * nb];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (FIRST) v[i][r] = pre(buf, j + r * nb);
+        else v[i][r] = buf[j + r * nb];
+      }
     }
   }
   __syncthreads();
   // twiddle angle per unit of k, in units of pi; k = j mod Ns by a mask when
-  // Ns is a power of two (every pass of radix 8, 4, 2)
+  // Ns is a power of two (every pass of radix 8, 4, 2), else from the FP32
+  // quotient (j + 1/2) / Ns, whose fraction stays >= 1/(2 Ns) from an
+  // integer while its rounding error is below 2^-22 j / Ns (j < 2^16)
   const double tscale = (double)sign * 2.0 / (double)(Ns * R);
   const bool pow2 = (Ns & (Ns - 1)) == 0;
+  const float rns = 1.0f / (float)Ns;
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int j = tid + i * NT;
     if (j < nb) {
-      const int k = pow2 ? (j & (Ns - 1)) : (j % Ns);
+      const int k = pow2 ? (j & (Ns - 1)) : (j - Ns * (int)(((float)j + 0.5f) * rns));
       if (Ns > 1) {
         const double2 w = expipi((double)k * tscale);
         double2 wr = w;
@@ -221,7 +243,13 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
       Dft<R>::run(v[i], sign);
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[i][r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (LAST) {
+          if (post(base + r * Ns, v[i][r])) buf[base + r * Ns] = v[i][r];
+        } else {
+          buf[base + r * Ns] = v[i][r];
+        }
+      }
     }
   }
   __syncthreads();
@@ -236,20 +264,37 @@ __device__ inline void stockham_pass(double2 *buf, int N, int Ns, int sign)
 // power-of-two length); 2 only 8.  The passes a plan cannot have are left
 // out of the kernel, which keeps its register count (and so its occupancy)
 // at what the passes it runs need.
+// Passes rad[s0 .. s1) starting from stride Ns; returns the stride after them.
 template <int NT, int E, int P2>
-__device__ void lds_fft(double2 *buf, int N, const int8_t *rad, int nrad, int sign)
+__device__ __forceinline__ int lds_fft(double2 *buf, int N, const int8_t *rad, int nrad, int sign, int s0 = 0,
+                       int Ns = 1, int s1 = -1)
 {
-  int Ns = 1, s = 0;
-  for (; s < nrad && rad[s] == 8; ++s, Ns *= 8) stockham_pass<NT, E, 8>(buf, N, Ns, sign);
+  if (s1 < 0) s1 = nrad;
+  int s = s0;
+  for (; s < s1 && rad[s] == 8; ++s, Ns *= 8) stockham_pass<NT, E, 8>(buf, N, Ns, sign);
   if constexpr (P2 < 2) {
-    if (s < nrad && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }
-    if (s < nrad && rad[s] == 2) { stockham_pass<NT, E, 2>(buf, N, Ns, sign); Ns *= 2; ++s; }
+    if (s < s1 && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }
+    if (s < s1 && rad[s] == 2) { stockham_pass<NT, E, 2>(buf, N, Ns, sign); Ns *= 2; ++s; }
   }
   if constexpr (P2 == 0) {
-    for (; s < nrad && rad[s] == 3; ++s, Ns *= 3) stockham_pass<NT, E, 3>(buf, N, Ns, sign);
-    for (; s < nrad && rad[s] == 5; ++s, Ns *= 5) stockham_pass<NT, E, 5>(buf, N, Ns, sign);
-    for (; s < nrad && rad[s] == 7; ++s, Ns *= 7) stockham_pass<NT, E, 7>(buf, N, Ns, sign);
+    for (; s < s1 && rad[s] == 3; ++s, Ns *= 3) stockham_pass<NT, E, 3>(buf, N, Ns, sign);
+    for (; s < s1 && rad[s] == 5; ++s, Ns *= 5) stockham_pass<NT, E, 5>(buf, N, Ns, sign);
+    for (; s < s1 && rad[s] == 7; ++s, Ns *= 7) stockham_pass<NT, E, 7>(buf, N, Ns, sign);
   }
+  return Ns;
+}
+
+// FFT of a Bluestein length M (a power of two >= 64) whose plan lists the
+// radices as 8, (8s, one 4 or 2), 8: the first and the last pass are of
+// radix 8, with `pre` and `post` applied.
+template <int NT, int E, int P2, typename Pre = PlainLoad, typename Post = PlainStore>
+__device__ __forceinline__ void bluestein_fft(double2 *buf, const RingFftPlan &P, int sign,
+                              const Pre &pre = Pre(), const Post &post = Post())
+{
+  const int M = P.M;
+  stockham_pass<NT, E, 8, true, false>(buf, M, 1, sign, pre);
+  const int Ns = lds_fft<NT, E, P2>(buf, M, P.rad, P.nrad, sign, 1, 8, P.nrad - 1);
+  stockham_pass<NT, E, 8, false, true>(buf, M, Ns, sign, PlainLoad(), post);
 }
 
 // Bluestein chirp h_j = e^{sign i pi j^2 / N} with the exact angle (table
@@ -273,7 +318,7 @@ __device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
     buf[j] = (d >= 0) ? chirp(d, N, -sign) : make_double2(0., 0.);
   }
   __syncthreads();
-  lds_fft<NT, E, 0>(buf, M, P.rad, P.nrad, -1);
+  bluestein_fft<NT, E, 1>(buf, P, -1);
 }
 
 // length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein;
@@ -281,7 +326,7 @@ __device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
 // convolution kernel (M entries, see bluestein_kernel), then the chirp
 // e^{-i pi j^2/N} (N entries)
 template <int NT, int E, bool BLUE, int P2>
-__device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const double2 *Bt)
+__device__ __forceinline__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
   if constexpr (!BLUE) {
     lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
@@ -289,24 +334,28 @@ __device__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const dou
   const int N = P.N, M = P.M;
   const double2 *Ct = Bt + M;
   auto ch = [&](int j) { const double2 c = Ct[j]; return sign > 0 ? conj(c) : c; };
-  for (int j = threadIdx.x; j < M; j += NT)
-    buf[j] = (j < N) ? cmul(buf[j], ch(j)) : make_double2(0., 0.);
-  __syncthreads();
-  lds_fft<NT, E, P2>(buf, M, P.rad, P.nrad, -1);
-  // the kernel of sign s is the conjugate of the kernel of sign -s with the
+  // forward transform of the chirp-weighted input, zero-padded to M (read
+  // in its first pass), times the kernel spectrum (in its last pass).  The
+  // kernel of sign s is the conjugate of the kernel of sign -s with the
   // input conjugated: FFT(conj(x))_k = conj(FFT(x)_{-k}); the table holds
   // the spectrum for the analysis sign -1
-  for (int j = threadIdx.x; j < M; j += NT) {
-    double2 b = Bt[j];
-    if (sign > 0) b = conj(Bt[j == 0 ? 0 : M - j]);
-    buf[j] = cmul(buf[j], b);
-  }
-  __syncthreads();
-  lds_fft<NT, E, P2>(buf, M, P.rad, P.nrad, +1);
+  auto chirped = [&](const double2 *b, int j) {
+    return j < N ? cmul(b[j], ch(j)) : make_double2(0., 0.);
+  };
+  auto kernel = [&](int j, double2 &v) {
+    const double2 b = sign > 0 ? conj(Bt[j == 0 ? 0 : M - j]) : Bt[j];
+    v = cmul(v, b);
+    return true;
+  };
+  bluestein_fft<NT, E, P2>(buf, P, -1, chirped, kernel);
+  // inverse transform; only outputs j < N are kept, times chirp and 1/M
   const double inv = 1.0 / M;
-  for (int j = threadIdx.x; j < N; j += NT)
-    buf[j] = cscale(cmul(buf[j], ch(j)), inv);
-  __syncthreads();
+  auto out = [&](int j, double2 &v) {
+    if (j >= N) return false;
+    v = cscale(cmul(v, ch(j)), inv);
+    return true;
+  };
+  bluestein_fft<NT, E, P2>(buf, P, +1, PlainLoad(), out);
   }
 }
 
@@ -689,14 +738,30 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
     if (a.dp) go(k_phase2ring_fft<S, B, ST, P2, double>);                  \
     else go(k_phase2ring_fft<S, B, ST, P2, float>);                        \
   } while (0)
+  // Bluestein lengths are powers of two: no Bluestein kernel of the radix
+  // set with 3, 5, 7
+  if constexpr (P2 == 0) {
+    if (blue) SHP_FAIL("ring FFT: Bluestein plan with an odd radix");
+  }
   if (synth) {
-    if (blue) { if (staged) SHP_SYN(true, true); else SHP_SYN(true, false); }
-    else { if (staged) SHP_SYN(false, true); else SHP_SYN(false, false); }
+    if constexpr (P2 > 0) {
+      if (blue) {
+        if (staged) SHP_SYN(true, true);
+        else SHP_SYN(true, false);
+      }
+    }
+    if (!blue) {
+      if (staged) SHP_SYN(false, true);
+      else SHP_SYN(false, false);
+    }
   } else {
-    if (blue) {
-      if (a.dp) go(k_ring2phase_fft<S, true, P2, double>);
-      else go(k_ring2phase_fft<S, true, P2, float>);
-    } else {
+    if constexpr (P2 > 0) {
+      if (blue) {
+        if (a.dp) go(k_ring2phase_fft<S, true, P2, double>);
+        else go(k_ring2phase_fft<S, true, P2, float>);
+      }
+    }
+    if (!blue) {
       if (a.dp) go(k_ring2phase_fft<S, false, P2, double>);
       else go(k_ring2phase_fft<S, false, P2, float>);
     }
