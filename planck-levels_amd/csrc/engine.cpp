@@ -145,6 +145,14 @@ struct NormPlan {
   ~NormPlan() { (void)hipFree(d); }
 };
 
+// word offsets of the BF16 pieces of each m (bs_pairs)
+struct BsPlan {
+  int lmax, mmax, spin, ncol;
+  int64_t total = 0;
+  int64_t *d = nullptr;
+  ~BsPlan() { (void)hipFree(d); }
+};
+
 struct AlmPlan {
   std::vector<char> key;
   int64_t *mvstart_by_m = nullptr;
@@ -214,10 +222,11 @@ struct Ctx {
   std::vector<std::unique_ptr<YlmPlan>> ylms;
   std::vector<std::unique_ptr<NormPlan>> norms;
   std::vector<std::unique_ptr<AlmPlan>> alms;
+  std::vector<std::unique_ptr<BsPlan>> bsplans;
   // work buffers per launch stream: calls on different streams do not share
   // them and may run concurrently
   struct StreamBufs {
-    DevBuf phase, almtmp, part, resid, lfac;
+    DevBuf phase, almtmp, part, resid, lfac, bsplit;
   };
   std::map<hipStream_t, std::unique_ptr<StreamBufs>> sbufs;
   StreamBufs &bufs(hipStream_t st)
@@ -592,6 +601,27 @@ static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
   return c.norms.back()->d;
 }
 
+static BsPlan &get_bsplan(Ctx &c, int lmax, int mmax, int spin, int ncol)
+{
+  for (auto &p : c.bsplans)
+    if (p->lmax == lmax && p->mmax == mmax && p->spin == spin && p->ncol == ncol) return *p;
+  auto bp = std::make_unique<BsPlan>();
+  bp->lmax = lmax;
+  bp->mmax = mmax;
+  bp->spin = spin;
+  bp->ncol = ncol;
+  std::vector<int64_t> base(mmax + 1);
+  int64_t t = 0;
+  for (int m = 0; m <= mmax; ++m) {
+    base[m] = t;
+    t += (int64_t)3 * ncol * bs_pairs(lmax, std::max(m, spin));
+  }
+  bp->total = t;
+  bp->d = upload(base);
+  c.bsplans.push_back(std::move(bp));
+  return *c.bsplans.back();
+}
+
 static int64_t *get_almplan(Ctx &c, const sharp_alm_info *a, int mmax)
 {
   std::vector<char> key;
@@ -795,6 +825,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   // the mixed syntheses read FP32 a_lm
   aa.tmp_f32 = (is_mixed(j.flags) && j.type == SHARP_ALM2MAP) ? 1 : 0;
   for (int i = 0; i < ncomp_alm; ++i) aa.alms.p[i] = dalm[i];
+  aa.bsplit = nullptr;
+  aa.bs_base = nullptr;
+  aa.bs_ncol = 0;
+  aa.spin = j.spin;
   aa.part = nullptr;
   aa.gmax = ml.gmax;
   aa.ngroups = (int)ml.hgmax.size();
@@ -837,6 +871,9 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   la.phase = phase;
   la.ngroups = (int)ml.hgmax.size();
   la.gmax = ml.gmax;
+  la.bsplit = nullptr;
+  la.bs_base = nullptr;
+  la.bs_ncol = 0;
 
   // triples (m, ring pair, l) with m <= mlim of the pair, and the flop model
   // constants of libsharp/sharp_core_inc2.c:171-173,440-442,583-585
@@ -1008,7 +1045,25 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
     stage_analysis(c, s, sb.part, st);
     return;
   }
+  if (mixed && mixed_contr(j.flags) == kContrBf16x3 && j.spin > 0) {
+    // the BF16 spin-s synthesis reads the a_lm split into BF16 pieces by the
+    // gather
+    const int ncol = 2 * s.la.ncomp_alm + 1;
+    BsPlan &bp = get_bsplan(c, j.lmax, j.mmax, j.spin, ncol);
+    unsigned *bs = static_cast<unsigned *>(sb.bsplit.get((size_t)bp.total * sizeof(unsigned)));
+    s.aa.bsplit = bs;
+    s.aa.bs_base = bp.d;
+    s.aa.bs_ncol = ncol;
+    s.la.bsplit = bs;
+    s.la.bs_base = bp.d;
+    s.la.bs_ncol = ncol;
+  }
   stage_gather(c, s, st);
+  if (s.aa.bsplit) {
+    const double bytes = (double)almtmp_elems(j) * sizeof(float2) +
+                         (double)get_bsplan(c, j.lmax, j.mmax, j.spin, s.aa.bs_ncol).total * 4;
+    timed(c, st, "alm_split", 0., bytes, [&] { launch_alm_split(s.aa, st); });
+  }
   if (!mixed) {
     stage_legendre_fp64(c, s, st);
   } else {
@@ -1622,9 +1677,10 @@ void sharp_hip_release(void)
   c.ylms.clear();
   c.norms.clear();
   c.alms.clear();
+  c.bsplans.clear();
   for (auto &kv : c.sbufs) {
     Ctx::StreamBufs &b = *kv.second;
-    for (DevBuf *d : {&b.phase, &b.almtmp, &b.part, &b.resid, &b.lfac}) d->release();
+    for (DevBuf *d : {&b.phase, &b.almtmp, &b.part, &b.resid, &b.lfac, &b.bsplit}) d->release();
   }
   c.sbufs.clear();
   c.stage_alm.release();

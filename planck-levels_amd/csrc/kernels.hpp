@@ -32,7 +32,24 @@ struct LegArgs {
                         // (mixed form) or double (FP64 matrix-core form)
   int64_t part_ld;      // floats per partial row
   int part_col0;        // float column of this launch's first transform
+  // BF16 spin-s synthesis: the a_lm split into BF16 pieces by the gather
+  // (bs_pairs), else nullptr
+  const unsigned *bsplit;
+  const int64_t *bs_base;
+  int bs_ncol;
 };
+
+// BF16 pieces of the FP32 a_lm of a BF16 spin-s synthesis, written by the
+// a_lm gather: for each m, real column c (2*ncomp_alm of them, then one zero
+// column) and piece q (h, m, l of the exact three-piece split) the rows
+// l = o, o+1, ... from o = max(m, spin) as pairs (o+2r, o+2r+1) in one 32-bit
+// word (row o+2r in the low half, the row past lmax zero), bs_pairs words
+// per (c, q), starting at word bs_base[m] + (3c + q) * bs_pairs.
+__host__ __device__ inline int bs_pairs(int lmax, int o)
+{
+  const int p = (lmax - o + 2) / 2;
+  return p > 0 ? p : 0;
+}
 
 // One tile of the persistent batched Legendre kernels: job index, transform
 // group (first transform, count, partial-sum column) and the tile's
@@ -151,11 +168,18 @@ struct AlmArgs {
   int ngroups, c0, npc;
   int64_t part_ld;
   int part_c0;
+  // BF16 spin-s synthesis: where launch_alm_split puts the BF16 pieces
+  // (bs_pairs), else nullptr
+  unsigned *bsplit;
+  const int64_t *bs_base;
+  int bs_ncol, spin;
 };
 // dst[c][k] = src[c][k] - dst[c][k] for k < n, c < ncomp (float or double)
 void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_t n,
                          bool dp, hipStream_t st);
 void launch_alm_gather(const AlmArgs &a, hipStream_t st);
 void launch_alm_scatter(const AlmArgs &a, hipStream_t st);
+// after the gather of a BF16 spin-s synthesis: the BF16 pieces (bs_pairs)
+void launch_alm_split(const AlmArgs &a, hipStream_t st);
 
 } // namespace shp

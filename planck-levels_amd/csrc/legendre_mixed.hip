@@ -630,25 +630,46 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
           flush(tot1, c1);
           flush(tot2, c2);
           nblk = 0;
-          int bxo[NCB], byo[NCB];
+          // B from the a_lm split into BF16 pieces by the gather (bs_pairs):
+          // X and Y columns of this lane (the zero column for unused ones), Y
+          // negated by its sign bit; rows l0+8kq.. are the 4 words from pair
+          // (l0+8kq-mhi)/2 (l - mhi is even here)
+          const unsigned *bsm = a.bsplit + a.bs_base[m];
+          const int np = bs_pairs(lmax, st.mhi);
+          const int czero = a.bs_ncol - 1, cbase = a.t0 * NALM * 2;
+          const unsigned *px[NCB], *py[NCB];
+          unsigned sg1[NCB], sg2[NCB];
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb) {
-            bxo[cb] = xo[cb] - kq * 2 * nc + 8 * kq * 2 * nc;
-            byo[cb] = yo[cb] - kq * 2 * nc + 8 * kq * 2 * nc;
+            const int col = cb * 16 + n, t = col >> 2, q = col & 3;
+            const bool ok = col < ncol;
+            int cx, cy;
+            bool neg;
+            if (!DERIV) {
+              cx = ok ? cbase + col : czero;
+              cy = ok ? cbase + (col ^ 3) : czero;
+              neg = ok && (q == 0 || q == 3);
+            } else {
+              cx = (ok && q < 2) ? cbase + 2 * t + q : czero;
+              cy = (ok && q >= 2) ? cbase + 2 * t + (q == 2 ? 1 : 0) : czero;
+              neg = ok && q == 3;
+            }
+            px[cb] = bsm + (int64_t)3 * cx * np + 4 * kq;
+            py[cb] = bsm + (int64_t)3 * cy * np + 4 * kq;
+            sg1[cb] = neg ? 0x80000000u : 0u;
+            sg2[cb] = neg ? 0x00008000u : 0u;
           }
           while (l + 31 <= lmax) {
             const int l0 = l;
-            float b1[NCB][8], b2[NCB][8];
+            const int r2 = (l0 - st.mhi) >> 1;
+            u32x4 PX[NCB][3], PY[NCB][3];
 #pragma unroll
-            for (int p = 0; p < 8; ++p) {
-              const float *row = arow + (int64_t)(l0 + p) * 2 * nc;
+            for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-              for (int cb = 0; cb < NCB; ++cb) {
-                const float bx = xs[cb] * row[bxo[cb]], by = ys[cb] * row[byo[cb]];
-                b1[cb][p] = (p & 1) ? by : bx;
-                b2[cb][p] = (p & 1) ? bx : by;
+              for (int q = 0; q < 3; ++q) {
+                __builtin_memcpy(&PX[cb][q], px[cb] + q * np + r2, sizeof(u32x4));
+                __builtin_memcpy(&PY[cb][q], py[cb] + q * np + r2, sizeof(u32x4));
               }
-            }
             float a1[32], a2[32];
 #pragma unroll
             for (int g4 = 0; g4 < 8; ++g4) {
@@ -686,9 +707,22 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 #pragma unroll
               for (int r = 0; r < 4; ++r) { a1[8 * r + p] = t1[r]; a2[8 * r + p] = t2[r]; }
             }
+            // accumulator 1: (X row 2d, Y row 2d+1) per word, accumulator 2:
+            // (Y row 2d, X row 2d+1), as split3 of the FP32 rows would give
             Bf3 B1[NCB], B2[NCB];
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) { B1[cb] = split3(b1[cb]); B2[cb] = split3(b2[cb]); }
+            for (int cb = 0; cb < NCB; ++cb) {
+              u32x4 w1[3], w2[3];
+#pragma unroll
+              for (int q = 0; q < 3; ++q) {
+                w1[q] = ((PY[cb][q] & 0xffff0000u) | (PX[cb][q] & 0x0000ffffu)) ^ sg1[cb];
+                w2[q] = ((PX[cb][q] & 0xffff0000u) | (PY[cb][q] & 0x0000ffffu)) ^ sg2[cb];
+              }
+              B1[cb] = Bf3{__builtin_bit_cast(bf16x8, w1[0]), __builtin_bit_cast(bf16x8, w1[1]),
+                           __builtin_bit_cast(bf16x8, w1[2])};
+              B2[cb] = Bf3{__builtin_bit_cast(bf16x8, w2[0]), __builtin_bit_cast(bf16x8, w2[1]),
+                           __builtin_bit_cast(bf16x8, w2[2])};
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               float v1[8], v2[8];
@@ -1460,6 +1494,7 @@ void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
     if (two) hipLaunchKernelGGL(k_msynth_spin0<2>, grid, block, 0, st, a, nsb);
     else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
   } else if (contr == kContrBf16x3) {
+    if (!a.bsplit) SHP_FAIL("BF16 spin synthesis without the split a_lm");
     // spin 0 keeps the FP32 matrix cores: with 16 columns the contraction is
     // too small a share for the splits to pay
     if (a.deriv1) {

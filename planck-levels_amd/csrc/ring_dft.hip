@@ -275,6 +275,54 @@ __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
     a.almtmp[o] = make_double2(re * f, im * f);
 }
 
+// BF16 pieces (bs_pairs) of the FP32 a_lm the gather wrote: one workgroup per
+// (m, 64 row pairs, 32 columns), through an LDS tile so that the reads of
+// almtmp rows and the writes of the piece words are both coalesced.  Column
+// 2*ncomp is the zero column.
+__global__ void __launch_bounds__(256) k_alm_split(AlmArgs a, int ntiles)
+{
+  __shared__ float tile[128][33];
+  const int m = blockIdx.y;
+  const int o0 = m < a.spin ? a.spin : m;
+  const int np = bs_pairs(a.lmax, o0);
+  const int r0 = (blockIdx.x % ntiles) * 64, c0 = (blockIdx.x / ntiles) * 32;
+  if (r0 >= np) return;
+  const int ncf = 2 * a.ncomp, nr = min(64, np - r0), ncol = min(32, a.bs_ncol - c0);
+  const float *rows = reinterpret_cast<const float *>(a.almtmp) +
+                      ((int64_t)tri_ofs(m, a.lmax) - m) * ncf;
+  for (int e = threadIdx.x; e < 128 * 32; e += 256) {
+    const int i = e >> 5, cc = e & 31, l = o0 + 2 * r0 + i, c = c0 + cc;
+    tile[i][cc] = (l <= a.lmax && c < ncf && i < 2 * nr) ? rows[(int64_t)l * ncf + c] : 0.f;
+  }
+  __syncthreads();
+  unsigned *w = a.bsplit + a.bs_base[m];
+  for (int e = threadIdx.x; e < ncol * 3 * 64; e += 256) {
+    const int p = e & 63, q = (e >> 6) % 3, cc = (e >> 6) / 3;
+    if (p >= nr) continue;
+    // piece q of the exact three-piece split (x = h + m + l, truncation)
+    auto piece = [&](float x) {
+      const unsigned xu = __float_as_uint(x);
+      if (q == 0) return xu >> 16;
+      const float r1 = x - __uint_as_float(xu & 0xffff0000u);
+      const unsigned ru = __float_as_uint(r1);
+      if (q == 1) return ru >> 16;
+      return __float_as_uint(r1 - __uint_as_float(ru & 0xffff0000u)) >> 16;
+    };
+    w[(int64_t)(3 * (c0 + cc) + q) * np + r0 + p] =
+        (piece(tile[2 * p + 1][cc]) << 16) | piece(tile[2 * p][cc]);
+  }
+}
+
+void launch_alm_split(const AlmArgs &a, hipStream_t s)
+{
+  const int ntiles = (bs_pairs(a.lmax, a.spin) + 63) / 64;
+  const int nchunks = (a.bs_ncol + 31) / 32;
+  if (ntiles <= 0) return;
+  const dim3 grid(ntiles * nchunks, a.mmax + 1);
+  hipLaunchKernelGGL(k_alm_split, grid, dim3(256), 0, s, a, ntiles);
+  check_launch("alm_split");
+}
+
 // components [c0, c0+npc); the values come from almtmp, or (part != nullptr)
 // from the partial sums of the ring-pair groups that reach m, added in FP64
 // in group order
