@@ -306,6 +306,9 @@ static int fft_plan(int rid, int n, RingFftPlan &pl)
     radices((int)M, rad);
     rad.erase(rad.begin());
     rad.push_back(8);
+    // M = 8192 (the class of 16 values per thread): 8, 16, 8, 8 (four LDS
+    // passes instead of five)
+    if (M == 8192) rad = {8, 16, 8, 8};
     pl.M = (int)M;
     need = M;
   }
@@ -390,10 +393,10 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     gp->npix_all += n;
     if (cl < 0) dft.push_back(rid);
     else {
-      // radix set: 2 only 8, 1 only 8, 4, 2, 0 any
+      // radix set: 2 only 8, 1 only 8, 4, 2 (and 16), 0 any
       int rs = 2;
       for (int q = 0; q < pl.nrad; ++q) {
-        if (pl.rad[q] == 4 || pl.rad[q] == 2) rs = std::min(rs, 1);
+        if (pl.rad[q] == 4 || pl.rad[q] == 2 || pl.rad[q] == 16) rs = std::min(rs, 1);
         else if (pl.rad[q] != 8) rs = 0;
       }
       cls[kFftKinds * cl + pl.bluestein + 2 * rs].push_back(pl);

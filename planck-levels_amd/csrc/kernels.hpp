@@ -118,7 +118,8 @@ void launch_phase2ring_dft(const RingArgs &a, const int *list, int nlist, hipStr
 void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStream_t st);
 
 // Per-ring FFT plan.  rad[] lists the Stockham radices of N (smooth lengths)
-// or of the Bluestein length M (8, then 8s and one 4 or 2, then a last 8).
+// or of the Bluestein length M (8, then 8s and one 4 or 2, then a last 8; at
+// M = 8192: 8, 16, 8, 8).
 struct RingFftPlan {
   int ring;        // slot*2 + hemi
   int n;           // pixels

@@ -118,6 +118,31 @@ template <> struct Dft<8> {
   }
 };
 
+template <> struct Dft<16> {
+  static __device__ inline void run(double2 *v, int sign)
+  {
+    double2 e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { e[k] = v[2 * k]; o[k] = v[2 * k + 1]; }
+    Dft<8>::run(e, sign);
+    Dft<8>::run(o, sign);
+    // o_k *= e^{sign 2 pi i k / 16}: cos and sin of pi/8
+    const double c = 0.923879532511286756128, s = 0.382683432365089771728, h = kSqrtHalf;
+    o[1] = cmul(o[1], make_double2(c, sign * s));
+    o[2] = cmul(o[2], make_double2(h, sign * h));
+    o[3] = cmul(o[3], make_double2(s, sign * c));
+    o[4] = mul_si(o[4], sign);
+    o[5] = cmul(o[5], make_double2(-s, sign * c));
+    o[6] = cmul(o[6], make_double2(-h, sign * h));
+    o[7] = cmul(o[7], make_double2(-c, sign * s));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 8] = csub(e[k], o[k]);
+    }
+  }
+};
+
 template <> struct Dft<3> {
   static __device__ inline void run(double2 *v, int sign)
   {
@@ -260,8 +285,8 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, int N, int Ns, int s
 // the passes are laid out in that order as straight-line code (a switch
 // inside one loop makes the register allocator keep every pass's values
 // live at once).
-// P2 (radix set of the plan): 0 any of 8, 4, 2, 3, 5, 7; 1 only 8, 4, 2 (a
-// power-of-two length); 2 only 8.  The passes a plan cannot have are left
+// P2 (radix set of the plan): 0 any of 8, 4, 2, 3, 5, 7; 1 only 8, 4, 2 and
+// (16 values per thread) 16: a power-of-two length; 2 only 8.  The passes a plan cannot have are left
 // out of the kernel, which keeps its register count (and so its occupancy)
 // at what the passes it runs need.
 // Passes rad[s0 .. s1) starting from stride Ns; returns the stride after them.
@@ -271,6 +296,9 @@ __device__ __forceinline__ int lds_fft(double2 *buf, int N, const int8_t *rad, i
 {
   if (s1 < 0) s1 = nrad;
   int s = s0;
+  // radix 16 (plans of 16 values per thread only, ahead of the 8s)
+  if constexpr (P2 < 2 && E >= 16)
+    for (; s < s1 && rad[s] == 16; ++s, Ns *= 16) stockham_pass<NT, E, 16>(buf, N, Ns, sign);
   for (; s < s1 && rad[s] == 8; ++s, Ns *= 8) stockham_pass<NT, E, 8>(buf, N, Ns, sign);
   if constexpr (P2 < 2) {
     if (s < s1 && rad[s] == 4) { stockham_pass<NT, E, 4>(buf, N, Ns, sign); Ns *= 4; ++s; }

@@ -129,6 +129,27 @@ def test_general_geometry(product, oracle, gpu, spin, jobtype):
     compare(product, oracle, _general_geom, tri(10), jobtype, spin, 2, O.DP)
 
 
+def _fft_class_geom(lib):
+    """Rings whose FFTs take the ring stage's plans up to length 8192:
+    Bluestein with M = 8192 (passes 8, 16, 8, 8; even nph 8156, 6010, 4100 and
+    odd nph 4093), M = 4096 (nph 2062) and M = 2048 (nph 1990), radix 8 only
+    (nph 8192) and mixed radices (nph 6000), with nph above 2 mmax + 1 (direct
+    fold) and below it (staged fold)."""
+    nph = np.array([8156, 4093, 6010, 4100, 8192, 6000, 2062, 1990])
+    theta = np.linspace(0.2, 2.9, len(nph))
+    stride = np.ones(len(nph), np.int64)
+    ofs = np.concatenate([[0], np.cumsum(nph)[:-1]]).astype(np.int64)
+    phi0 = np.linspace(0.0, 0.7, len(nph))
+    wgt = np.linspace(0.5, 1.5, len(nph))
+    return lib.general_geom(nph, ofs, stride, phi0, theta, wgt)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
+def test_ring_fft_classes(product, oracle, gpu, spin, jobtype):
+    compare(product, oracle, _fft_class_geom, tri(2100), jobtype, spin, 1, O.DP)
+
+
 def test_packed(product, oracle, gpu):
     def packed(lib):
         lmax = 12
