@@ -1061,12 +1061,10 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
     s.la.bs_base = bp.d;
     s.la.bs_ncol = ncol;
   }
-  stage_gather(c, s, st);
-  if (s.aa.bsplit) {
-    const double bytes = (double)almtmp_elems(j) * sizeof(float2) +
-                         (double)get_bsplan(c, j.lmax, j.mmax, j.spin, s.aa.bs_ncol).total * 4;
-    timed(c, st, "alm_split", 0., bytes, [&] { launch_alm_split(s.aa, st); });
-  }
+  if (s.aa.bsplit)
+    timed(c, st, "alm_gather", 0., s.alm_bytes, [&] { launch_alm_gather_split(s.aa, st); });
+  else
+    stage_gather(c, s, st);
   if (!mixed) {
     stage_legendre_fp64(c, s, st);
   } else {

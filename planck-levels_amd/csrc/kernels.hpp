@@ -169,8 +169,8 @@ struct AlmArgs {
   int ngroups, c0, npc;
   int64_t part_ld;
   int part_c0;
-  // BF16 spin-s synthesis: where launch_alm_split puts the BF16 pieces
-  // (bs_pairs), else nullptr
+  // BF16 spin-s synthesis: where launch_alm_gather_split puts the BF16
+  // pieces (bs_pairs), else nullptr
   unsigned *bsplit;
   const int64_t *bs_base;
   int bs_ncol, spin;
@@ -180,7 +180,8 @@ void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_
                          bool dp, hipStream_t st);
 void launch_alm_gather(const AlmArgs &a, hipStream_t st);
 void launch_alm_scatter(const AlmArgs &a, hipStream_t st);
-// after the gather of a BF16 spin-s synthesis: the BF16 pieces (bs_pairs)
-void launch_alm_split(const AlmArgs &a, hipStream_t st);
+// the gather of a BF16 spin-s synthesis: FP32 rows and their BF16 pieces
+// (bs_pairs)
+void launch_alm_gather_split(const AlmArgs &a, hipStream_t st);
 
 } // namespace shp

@@ -251,6 +251,24 @@ void launch_ring2phase_dft(const RingArgs &a, const int *list, int nlist, hipStr
 // element e of row m is l = m + e / ncomp, component e % ncomp, so the
 // canonical rows are read and written contiguously.
 // ---------------------------------------------------------------------------
+// a_{l,m} of component comp from the caller's array, times norm_l (and sqrt 2
+// for REAL_HARMONICS at m = 0); only the real part for m = 0
+// (libsharp/sharp.c:481-533)
+template <typename T>
+__device__ inline double2 gather_value(const AlmArgs &a, int m, int l, int comp)
+{
+  const bool realonly = a.packed && m == 0;
+  int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
+  if (!a.packed) ofs *= 2;
+  if (!realonly) stride *= 2;
+  const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
+  const void *p = a.alms.p[comp];
+  const double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
+  double f = a.norm_l[l];
+  if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
+  return make_double2(re * f, im * f);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
 {
@@ -259,42 +277,55 @@ __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
   const int nc = a.ncomp;
   if (e >= (int64_t)(a.lmax - m + 1) * nc) return;
   const int l = m + (int)(e / nc), comp = (int)(e % nc);
-  const bool realonly = a.packed && m == 0;
-  int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
-  if (!a.packed) ofs *= 2;
-  if (!realonly) stride *= 2;
-  const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
-  const void *p = a.alms.p[comp];
-  double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
-  double f = a.norm_l[l];
-  if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
+  const double2 v = gather_value<T>(a, m, l, comp);
   const int64_t o = (int64_t)tri_ofs(m, a.lmax) * nc + e;
   if (a.tmp_f32)
-    reinterpret_cast<float2 *>(a.almtmp)[o] = make_float2((float)(re * f), (float)(im * f));
+    reinterpret_cast<float2 *>(a.almtmp)[o] = make_float2((float)v.x, (float)v.y);
   else
-    a.almtmp[o] = make_double2(re * f, im * f);
+    a.almtmp[o] = v;
 }
 
-// BF16 pieces (bs_pairs) of the FP32 a_lm the gather wrote: one workgroup per
-// (m, 64 row pairs, 32 columns), through an LDS tile so that the reads of
-// almtmp rows and the writes of the piece words are both coalesced.  Column
-// 2*ncomp is the zero column.
-__global__ void __launch_bounds__(256) k_alm_split(AlmArgs a, int ntiles)
+// Gather of a BF16 spin-s synthesis: the FP32 canonical rows as k_alm_gather
+// writes them, and their BF16 pieces (bs_pairs).  One workgroup per (m, 64
+// row pairs from o = max(m, spin), 16 components) goes through an LDS tile,
+// so that the reads of the caller's a_lm (along l), the FP32 rows (along the
+// columns) and the piece words (along the pairs) are all coalesced.  The
+// rows m <= l < o (m < spin) get FP32 rows only; column 2*ncomp of the pieces
+// is the zero column.
+template <typename T>
+__global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
 {
   __shared__ float tile[128][33];
   const int m = blockIdx.y;
   const int o0 = m < a.spin ? a.spin : m;
   const int np = bs_pairs(a.lmax, o0);
-  const int r0 = (blockIdx.x % ntiles) * 64, c0 = (blockIdx.x / ntiles) * 32;
+  const int tix = blockIdx.x % ntiles, chunk = blockIdx.x / ntiles;
+  const int r0 = tix * 64, c0 = chunk * 16, nc = a.ncomp, ncf = 2 * nc;
+  float *rows = reinterpret_cast<float *>(a.almtmp) + ((int64_t)tri_ofs(m, a.lmax) - m) * ncf;
+  if (tix == 0) {
+    for (int e = threadIdx.x; e < (o0 - m) * 16; e += 256) {
+      const int l = m + e / 16, comp = c0 + e % 16;
+      if (l > a.lmax || comp >= nc) continue;
+      const double2 v = gather_value<T>(a, m, l, comp);
+      rows[(int64_t)l * ncf + 2 * comp] = (float)v.x;
+      rows[(int64_t)l * ncf + 2 * comp + 1] = (float)v.y;
+    }
+  }
   if (r0 >= np) return;
-  const int ncf = 2 * a.ncomp, nr = min(64, np - r0), ncol = min(32, a.bs_ncol - c0);
-  const float *rows = reinterpret_cast<const float *>(a.almtmp) +
-                      ((int64_t)tri_ofs(m, a.lmax) - m) * ncf;
-  for (int e = threadIdx.x; e < 128 * 32; e += 256) {
-    const int i = e >> 5, cc = e & 31, l = o0 + 2 * r0 + i, c = c0 + cc;
-    tile[i][cc] = (l <= a.lmax && c < ncf && i < 2 * nr) ? rows[(int64_t)l * ncf + c] : 0.f;
+  const int nr = min(64, np - r0), ncol = min(32, a.bs_ncol - 2 * c0);
+  const int lb = o0 + 2 * r0;
+  for (int e = threadIdx.x; e < 128 * 16; e += 256) {
+    const int i = e & 127, cc = e >> 7, l = lb + i, comp = c0 + cc;
+    double2 v = make_double2(0., 0.);
+    if (i < 2 * nr && l <= a.lmax && comp < nc) v = gather_value<T>(a, m, l, comp);
+    tile[i][2 * cc] = (float)v.x;
+    tile[i][2 * cc + 1] = (float)v.y;
   }
   __syncthreads();
+  for (int e = threadIdx.x; e < 128 * 32; e += 256) {
+    const int i = e >> 5, cc = e & 31, l = lb + i, c = 2 * c0 + cc;
+    if (i < 2 * nr && l <= a.lmax && c < ncf) rows[(int64_t)l * ncf + c] = tile[i][cc];
+  }
   unsigned *w = a.bsplit + a.bs_base[m];
   for (int e = threadIdx.x; e < ncol * 3 * 64; e += 256) {
     const int p = e & 63, q = (e >> 6) % 3, cc = (e >> 6) / 3;
@@ -308,19 +339,23 @@ __global__ void __launch_bounds__(256) k_alm_split(AlmArgs a, int ntiles)
       if (q == 1) return ru >> 16;
       return __float_as_uint(r1 - __uint_as_float(ru & 0xffff0000u)) >> 16;
     };
-    w[(int64_t)(3 * (c0 + cc) + q) * np + r0 + p] =
+    w[(int64_t)(3 * (2 * c0 + cc) + q) * np + r0 + p] =
         (piece(tile[2 * p + 1][cc]) << 16) | piece(tile[2 * p][cc]);
   }
 }
 
-void launch_alm_split(const AlmArgs &a, hipStream_t s)
+void launch_alm_gather_split(const AlmArgs &a, hipStream_t s)
 {
-  const int ntiles = (bs_pairs(a.lmax, a.spin) + 63) / 64;
+  // tiles from the longest m (pairs from max(m, spin)); chunks of 16
+  // components, the last one holding the zero column
+  const int ntiles = std::max(1, (bs_pairs(a.lmax, a.spin) + 63) / 64);
   const int nchunks = (a.bs_ncol + 31) / 32;
-  if (ntiles <= 0) return;
   const dim3 grid(ntiles * nchunks, a.mmax + 1);
-  hipLaunchKernelGGL(k_alm_split, grid, dim3(256), 0, s, a, ntiles);
-  check_launch("alm_split");
+  if (a.flags & SHARP_DP)
+    hipLaunchKernelGGL(k_alm_gather_split<double>, grid, dim3(256), 0, s, a, ntiles);
+  else
+    hipLaunchKernelGGL(k_alm_gather_split<float>, grid, dim3(256), 0, s, a, ntiles);
+  check_launch("alm_gather_split");
 }
 
 // components [c0, c0+npc); the values come from almtmp, or (part != nullptr)
