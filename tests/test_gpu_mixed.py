@@ -95,3 +95,32 @@ def test_larger_vs_fp64(product, gpu, spin, form):
         print("mixed vs fp64", jt, spin, max(errs), min(errs))
         assert max(errs) < TOLMIX, errs
         assert min(errs) > 1e-12, errs
+
+
+def _packed96(lib):
+    """Packed layout (real-only m = 0 block), lmax 96."""
+    lmax = 96
+    mval = np.arange(lmax + 1)
+    mvs = np.zeros(lmax + 1, np.int64)
+    pos = lmax + 1
+    for m in range(1, lmax + 1):
+        mvs[m] = pos - 2 * m
+        pos += 2 * (lmax - m + 1)
+    return lib.general_alm(lmax, mval, mvs, 1, O.PACKED)
+
+
+@pytest.mark.parametrize("case", ["second_launch", "deriv1", "packed_add"])
+def test_whole_groups(product, oracle, gpu, case, form):
+    """lmax 96, so that the unscaled part has whole 32-l groups (the BF16
+    synthesis reads its a_lm pre-split there): transforms 8 and up (a second
+    launch, its first column past the first 32), DERIV1 (its own column
+    pattern), and a packed a_lm layout with SHARP_ADD through the gather."""
+    if case == "second_launch":
+        for jt in (O.ALM2MAP, O.MAP2ALM):
+            compare_mixed(product, oracle, iso("gauss", 97, 193), tri(96), jt, 2, 10, form=form)
+    elif case == "deriv1":
+        compare_mixed(product, oracle, healpix(32), tri(96), O.DERIV1, 1, 3, form=form)
+    else:
+        for jt in (O.ALM2MAP, O.MAP2ALM):
+            compare_mixed(product, oracle, healpix(32), _packed96, jt, 2, 2,
+                          flags=O.DP | O.ADD, form=form)
