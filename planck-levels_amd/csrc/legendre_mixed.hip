@@ -31,8 +31,9 @@
 // results do not depend on scheduling.
 //
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2;
-// xcd_map keeps all workgroups of one m (analysis) or m-block (synthesis) on
-// one XCD, so the recurrence tables and a_lm rows of that m are fetched into
+// xcd_map keeps all workgroups of one m-block (synthesis) on one XCD, and
+// xcd_map_mblock those of the 8 m of one phase block (analysis), so the
+// recurrence tables, a_lm rows and phase lines of those m are fetched into
 // one L2 only.
 
 #include <hip/hip_runtime.h>
@@ -146,6 +147,25 @@ __device__ inline bool xcd_map(int id, int nmajor, int nminor, int &major, int &
 static inline int xcd_grid(int nmajor, int nminor)
 {
   return ((nmajor + 7) / 8) * 8 * nminor;
+}
+
+// Analysis: as xcd_map, but the 8 m of one phase block (m / 8: a phase cache
+// line holds the 8 m of a ring, phase_index) go to one XCD, issued next to
+// each other for each ring group, so that they share the block's lines in
+// that L2; blocks go round-robin over the XCDs.
+__device__ inline bool xcd_map_mblock(int id, int nmajor, int nminor, int &major, int &minor)
+{
+  const int xcd = id & 7, k = id >> 3, per = 8 * nminor;
+  const int blk = (k / per) * 8 + xcd, r = k % per;
+  minor = r / 8;
+  major = blk * 8 + r % 8;
+  return major < nmajor;
+}
+
+static inline int xcd_grid_mblock(int nmajor, int nminor)
+{
+  const int nblk = (nmajor + 7) / 8;
+  return ((nblk + 7) / 8) * 64 * nminor;
 }
 
 constexpr int kFlushBlocks = 8;   // K slices per FP32 partial sum
@@ -1387,7 +1407,7 @@ __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
   __shared__ __align__(16) T lds[kMixWaves0][kSlice];
   __shared__ double cbuf[2][64];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
+  if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
     manal_spin0_tile<T, NCB, CT>(a, m, g, lds, cbuf);
 }
 
@@ -1397,7 +1417,7 @@ __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
   __shared__ __align__(16) T lds[kMixWavesS][2][kSlice];
   __shared__ double cbuf[2][96];
   int m, g;
-  if (xcd_map(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
+  if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
     manal_spin_tile<T, NCB, CT>(a, m, g, lds, cbuf);
 }
 
@@ -1550,7 +1570,7 @@ void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items
 void launch_manal(const LegArgs &a, int contr, hipStream_t st)
 {
   const bool f64 = contr == kContrF64, bf = contr == kContrBf16x3;
-  const dim3 grid(xcd_grid(a.mmax + 1, a.ngroups));
+  const dim3 grid(xcd_grid_mblock(a.mmax + 1, a.ngroups));
   const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
   if (a.ntl < 1 || cols > (f64 ? 16 : 32)) SHP_FAIL("bad transform group");
