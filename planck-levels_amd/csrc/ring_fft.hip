@@ -397,6 +397,16 @@ __device__ inline double2 group_sum(double2 v, int G)
   return v;
 }
 
+// Workgroup b transforms plan pair_plan(b): plans 2q and 2q+1 (the northern
+// and southern ring of a pair: same length, same Bluestein block) go to one
+// XCD, eight workgroups apart, so that they share that block in its L2.  A
+// grid of 16k workgroups covers plans 0 .. 16k-1 once.
+__device__ inline int pair_plan(int b)
+{
+  const int xcd = b & 7, k = b >> 3;
+  return (((k >> 1) << 3) + xcd) * 2 + (k & 1);
+}
+
 // threads per ring: S/8 (8 values per thread and pass) within [256, 512];
 // small rings still read all mmax+1 phases, which wants many threads
 template <int S> constexpr int kThreads = (S / 8 < 256) ? 256 : ((S / 8 < 512) ? S / 8 : 512);
@@ -429,7 +439,7 @@ __device__ inline void batched(int lo, int hi, L &&load, C &&consume)
 // ===========================================================================
 template <int S, bool BLUE, bool STAGED, int P2, typename T>
 __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
-                                                          int comps_per_block, int vcap,
+                                                          int nplans, int comps_per_block, int vcap,
                                                           const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
@@ -437,7 +447,9 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
   extern __shared__ double2 smem[];
   double2 *buf = smem, *V = smem + S;
   __shared__ double2 nyq;
-  const RingFftPlan &P = plans[blockIdx.x];
+  const int pi = pair_plan(blockIdx.x);
+  if (pi >= nplans) return;
+  const RingFftPlan &P = plans[pi];
   const int tid = threadIdx.x;
   const int slot = P.ring >> 1, hemi = P.ring & 1;
   const RingDev r = a.ring[P.ring];
@@ -601,14 +613,16 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
 // ===========================================================================
 template <int S, bool BLUE, int P2, typename T>
 __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
-                                                          int comps_per_block, int,
+                                                          int nplans, int comps_per_block, int,
                                                           const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
   extern __shared__ double2 smem[];
   double2 *buf = smem;
   __shared__ double2 nyq;
-  const RingFftPlan &P = plans[blockIdx.x];
+  const int pi = pair_plan(blockIdx.x);
+  if (pi >= nplans) return;
+  const RingFftPlan &P = plans[pi];
   const int tid = threadIdx.x;
   const int slot = P.ring >> 1, hemi = P.ring & 1;
   const RingDev r = a.ring[P.ring];
@@ -741,7 +755,7 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   // split components over blocks only when there are few rings
   int cpb = a.ncomp;
   while (cpb > 1 && (int64_t)nplans * ((a.ncomp + cpb - 1) / cpb) < 2048) cpb = (cpb + 1) / 2;
-  const dim3 grid(nplans, (a.ncomp + cpb - 1) / cpb);
+  const dim3 grid((nplans + 15) / 16 * 16, (a.ncomp + cpb - 1) / cpb);
   const dim3 block(kThreads<S>);
   size_t need = 0;
   for (int i = 0; i < nplans; ++i) need = std::max(need, plan_need(hplans[i]));
@@ -759,7 +773,7 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   const size_t lds = fbytes + (size_t)vcap * sizeof(double2);
   auto go = [&](auto kern) {
     set_lds(kern, lds);
-    hipLaunchKernelGGL(kern, grid, block, lds, st, a, plans, cpb, vcap, btab);
+    hipLaunchKernelGGL(kern, grid, block, lds, st, a, plans, nplans, cpb, vcap, btab);
   };
 #define SHP_SYN(B, ST)                                                     \
   do {                                                                     \
