@@ -387,8 +387,9 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     all.push_back(rid);
     RingFftPlan pl;
     int cl = fft_plan(rid, n, pl);
-    // the class's FFT buffer plus room to stage 512 phases (ring_fft.hip)
-    if (cl >= 0 && (size_t)((512 << cl) + 512) * sizeof(double2) + 256 > (size_t)c.max_lds)
+    // the class's FFT buffer (with its pad of one entry in 8) plus room to
+    // stage 512 phases (ring_fft.hip)
+    if (cl >= 0 && (size_t)((512 << cl) + (64 << cl) + 512) * sizeof(double2) + 256 > (size_t)c.max_lds)
       cl = -1;
     gp->npix_all += n;
     if (cl < 0) dft.push_back(rid);

@@ -55,6 +55,16 @@ __device__ inline double2 cmul(double2 a, double2 b)
 }
 __device__ inline double2 conj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// The FFT buffer in LDS with a 16-byte pad after every 8 entries: entry i at
+// i + i/8.  The first Stockham pass writes the outputs of a butterfly to
+// consecutive entries, R entries apart from lane to lane; unpadded, the lanes
+// of a radix-8 pass then land on the same banks.
+__host__ __device__ constexpr int lds_pad(int n) { return n + (n >> 3); }
+struct Lds {
+  double2 *p;
+  __device__ double2 &operator[](int i) const { return p[i + (i >> 3)]; }
+};
+
 // multiply by sign*i
 __device__ inline double2 mul_si(double2 a, int sign)
 {
@@ -217,7 +227,7 @@ template <> struct Dft<7> : DftOdd<7, Tw7> {};
 // updates v and returns whether buf[i] is written).  The Bluestein
 // convolution folds its pointwise products into them.
 struct PlainLoad {
-  __device__ double2 operator()(const double2 *b, int i) const { return b[i]; }
+  __device__ double2 operator()(Lds b, int i) const { return b[i]; }
 };
 struct PlainStore {
   __device__ bool operator()(int, double2 &) const { return true; }
@@ -225,7 +235,7 @@ struct PlainStore {
 
 template <int NT, int E, int R, bool FIRST = false, bool LAST = false,
           typename Pre = PlainLoad, typename Post = PlainStore>
-__device__ __forceinline__ void stockham_pass(double2 *buf, int N, int Ns, int sign,
+__device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
                                      const Pre &pre = Pre(), const Post &post = Post())
 {
   constexpr int BPT = (E + R - 1) / R;
@@ -291,7 +301,7 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, int N, int Ns, int s
 // at what the passes it runs need.
 // Passes rad[s0 .. s1) starting from stride Ns; returns the stride after them.
 template <int NT, int E, int P2>
-__device__ __forceinline__ int lds_fft(double2 *buf, int N, const int8_t *rad, int nrad, int sign, int s0 = 0,
+__device__ __forceinline__ int lds_fft(Lds buf, int N, const int8_t *rad, int nrad, int sign, int s0 = 0,
                        int Ns = 1, int s1 = -1)
 {
   if (s1 < 0) s1 = nrad;
@@ -316,7 +326,7 @@ __device__ __forceinline__ int lds_fft(double2 *buf, int N, const int8_t *rad, i
 // radices as 8, (8s, one 4 or 2), 8: the first and the last pass are of
 // radix 8, with `pre` and `post` applied.
 template <int NT, int E, int P2, typename Pre = PlainLoad, typename Post = PlainStore>
-__device__ __forceinline__ void bluestein_fft(double2 *buf, const RingFftPlan &P, int sign,
+__device__ __forceinline__ void bluestein_fft(Lds buf, const RingFftPlan &P, int sign,
                               const Pre &pre = Pre(), const Post &post = Post())
 {
   const int M = P.M;
@@ -336,7 +346,7 @@ __device__ inline double2 chirp(int64_t j, int N, int sign)
 // FFT of the convolution kernel b_d = e^{-sign i pi d^2 / N}, |d| < N,
 // wrapped to length M, left in buf[0..M)
 template <int NT, int E>
-__device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
+__device__ void bluestein_kernel(Lds buf, const RingFftPlan &P, int sign)
 {
   const int N = P.N, M = P.M;
   for (int j = threadIdx.x; j < M; j += NT) {
@@ -354,7 +364,7 @@ __device__ void bluestein_kernel(double2 *buf, const RingFftPlan &P, int sign)
 // convolution kernel (M entries, see bluestein_kernel), then the chirp
 // e^{-i pi j^2/N} (N entries)
 template <int NT, int E, bool BLUE, int P2>
-__device__ __forceinline__ void ring_dft(double2 *buf, const RingFftPlan &P, int sign, const double2 *Bt)
+__device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
   if constexpr (!BLUE) {
     lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
@@ -367,7 +377,7 @@ __device__ __forceinline__ void ring_dft(double2 *buf, const RingFftPlan &P, int
   // kernel of sign s is the conjugate of the kernel of sign -s with the
   // input conjugated: FFT(conj(x))_k = conj(FFT(x)_{-k}); the table holds
   // the spectrum for the analysis sign -1
-  auto chirped = [&](const double2 *b, int j) {
+  auto chirped = [&](Lds b, int j) {
     return j < N ? cmul(b[j], ch(j)) : make_double2(0., 0.);
   };
   auto kernel = [&](int j, double2 &v) {
@@ -410,7 +420,6 @@ __device__ inline int pair_plan(int b)
 // threads per ring: S/8 (8 values per thread and pass) within [256, 512];
 // small rings still read all mmax+1 phases, which wants many threads
 template <int S> constexpr int kThreads = (S / 8 < 256) ? 256 : ((S / 8 < 512) ? S / 8 : 512);
-#define SHP_FFT_OCC
 
 // Runs consume(i, load(i)) for i = lo+tid, lo+tid+NT, ... < hi with U loads
 // in flight per thread (all U are issued before the first is used).
@@ -437,15 +446,22 @@ __device__ inline void batched(int lo, int hi, L &&load, C &&consume)
 // ===========================================================================
 // synthesis: phase -> ring
 // ===========================================================================
+// minimum waves per SIMD: the radix-8 (equatorial) 4096 class, unstaged,
+// runs two 512-thread workgroups per CU (72 KB of LDS each), 4 waves per
+// SIMD, which caps its registers at 128 (it needs no more)
+template <int S, bool BLUE, bool STAGED, int P2>
+constexpr int kMinBlocks = (S == 4096 && !BLUE && !STAGED && P2 == 2) ? 4 : 1;
+
 template <int S, bool BLUE, bool STAGED, int P2, typename T>
-__global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
+__global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
                                                           int nplans, int comps_per_block, int vcap,
                                                           const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
   // buf: the ring's FFT buffer; V (STAGED): up to vcap weighted, rotated phases
   extern __shared__ double2 smem[];
-  double2 *buf = smem, *V = smem + S;
+  const Lds buf{smem};
+  double2 *V = smem + lds_pad(S);
   __shared__ double2 nyq;
   const int pi = pair_plan(blockIdx.x);
   if (pi >= nplans) return;
@@ -612,13 +628,13 @@ __global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_phase2ring_fft(Ring
 // analysis: ring -> phase
 // ===========================================================================
 template <int S, bool BLUE, int P2, typename T>
-__global__ void __launch_bounds__(kThreads<S>) SHP_FFT_OCC k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
+__global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>)) k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
                                                           int nplans, int comps_per_block, int,
                                                           const double2 *btab)
 {
   constexpr int NT = kThreads<S>, E = S / NT;
   extern __shared__ double2 smem[];
-  double2 *buf = smem;
+  const Lds buf{smem};
   __shared__ double2 nyq;
   const int pi = pair_plan(blockIdx.x);
   if (pi >= nplans) return;
@@ -728,8 +744,9 @@ __global__ void __launch_bounds__(kThreads<S>) k_bluestein_table(const RingFftPl
   constexpr int NT = kThreads<S>, E = S / NT;
   extern __shared__ double2 smem[];
   const RingFftPlan &P = plans[blockIdx.x];
-  bluestein_kernel<NT, E>(smem, P, -1);
-  for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = smem[j];
+  const Lds buf{smem};
+  bluestein_kernel<NT, E>(buf, P, -1);
+  for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = buf[j];
   for (int j = threadIdx.x; j < P.N; j += NT) btab[P.boff + P.M + j] = chirp(j, P.N, -1);
 }
 
@@ -759,8 +776,8 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   const dim3 block(kThreads<S>);
   size_t need = 0;
   for (int i = 0; i < nplans; ++i) need = std::max(need, plan_need(hplans[i]));
-  // the staging area V starts after the class's S-entry buffer
-  const size_t fbytes = (staged ? (size_t)S : need) * sizeof(double2);
+  // the staging area V starts after the class's S-entry buffer (padded)
+  const size_t fbytes = (size_t)lds_pad(staged ? S : (int)need) * sizeof(double2);
   // staged synthesis keeps up to vcap phases next to the FFT buffer: the
   // whole ring's mmax+1 when the LDS allows, else it loops
   int vcap = 0;
@@ -869,7 +886,7 @@ static void table_class(const RingFftPlan *plans, const RingFftPlan *hplans, int
   if (np <= 0) return;
   size_t need = 0;
   for (int i = 0; i < np; ++i) need = std::max(need, plan_need(hplans[i]));
-  const size_t lds = need * sizeof(double2);
+  const size_t lds = (size_t)lds_pad((int)need) * sizeof(double2);
   set_lds(k_bluestein_table<S>, lds);
   hipLaunchKernelGGL(k_bluestein_table<S>, dim3(np), dim3(kThreads<S>), lds, st, plans, btab);
   check_launch("bluestein_table");
