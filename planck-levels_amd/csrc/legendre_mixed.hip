@@ -839,15 +839,6 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 // ===========================================================================
 // analysis
 // ===========================================================================
-// the double held by lane `src` (a compile-time constant after unrolling) in
-// every lane, through two v_readlane_b32: a wave-uniform (scalar) value
-__device__ inline double bcast_d(double v, int src)
-{
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 constexpr int kLS = 33;    // LDS row stride (floats) of the lambda slices: odd,
                            // so row-wise stores and the transposed MFMA loads
                            // hit distinct banks
