@@ -1394,6 +1394,51 @@ static void batch_tune(Ctx &c, BatchPlan &b, int ncu)
 
 using namespace shp;
 
+// Whether sorted intervals [first, second) cover [lo, hi] once, without gaps.
+static bool covers(std::vector<std::pair<int64_t, int64_t>> &iv, int64_t lo, int64_t hi)
+{
+  if (iv.empty()) return false;
+  std::sort(iv.begin(), iv.end());
+  int64_t end = lo;
+  for (const auto &x : iv) {
+    if (x.first != end) return false;
+    end = x.second;
+  }
+  return end == hi + 1;
+}
+
+// Every entry of the map range [lo, hi] (pixel units) belongs to one ring:
+// then a map that is only written needs no upload before the download.
+static bool map_dense(const sharp_geom_info *g, int64_t lo, int64_t hi)
+{
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  for (int i = 0; i < g->npairs; ++i)
+    for (const sharp_ringinfo *r : {&g->pair[i].r1, &g->pair[i].r2}) {
+      if (r->nph <= 0) continue;
+      if (r->stride != 1) return false;
+      iv.emplace_back(r->ofs, r->ofs + r->nph);
+    }
+  return covers(iv, lo, hi);
+}
+
+// Every entry of the a_lm range [lo, hi] (real units) is an a_lm element.
+static bool alm_dense(const sharp_alm_info *a, int64_t lo, int64_t hi)
+{
+  const bool packed = a->flags & SHARP_PACKED;
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  for (int mi = 0; mi < a->nm; ++mi) {
+    const int m = a->mval[mi];
+    if (m > a->lmax) continue;
+    const bool realonly = packed && m == 0;
+    int64_t ofs = a->mvstart[mi], stride = a->stride;
+    if (!packed) ofs *= 2;
+    if (!realonly) stride *= 2;
+    if (stride != (realonly ? 1 : 2)) return false;
+    iv.emplace_back(ofs + (int64_t)m * stride, ofs + (int64_t)(a->lmax + 1) * stride);
+  }
+  return covers(iv, lo, hi);
+}
+
 extern "C" {
 
 void sharp_execute(sharp_jobtype type, int spin, void *alm, void *map,
@@ -1418,15 +1463,23 @@ void sharp_execute(sharp_jobtype type, int spin, void *alm, void *map,
   void **halm = static_cast<void **>(alm), **hmap = static_cast<void **>(map);
   std::vector<void *> dalm(ncomp_alm), dmap(ncomp_map);
   hipStream_t st = c.stream;
+  // the output array is uploaded only when the download could otherwise
+  // overwrite host entries the transform does not write (SHARP_ADD, or gaps
+  // in its range)
+  const bool synth = j.type == SHARP_ALM2MAP, add = (j.flags & SHARP_ADD) != 0;
+  const bool up_alm = synth || add || !alm_dense(alm_info, alo, ahi);
+  const bool up_map = !synth || add || !map_dense(geom_info, mlo, mhi);
   for (int i = 0; i < ncomp_alm; ++i) {
     dalm[i] = da + i * an * esz;
-    HIP_CHECK(hipMemcpyAsync(dalm[i], static_cast<char *>(halm[i]) + alo * (int64_t)esz,
-                             an * esz, hipMemcpyHostToDevice, st));
+    if (up_alm)
+      HIP_CHECK(hipMemcpyAsync(dalm[i], static_cast<char *>(halm[i]) + alo * (int64_t)esz,
+                               an * esz, hipMemcpyHostToDevice, st));
   }
   for (int i = 0; i < ncomp_map; ++i) {
     dmap[i] = dm + i * mn * esz;
-    HIP_CHECK(hipMemcpyAsync(dmap[i], static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz,
-                             mn * esz, hipMemcpyHostToDevice, st));
+    if (up_map)
+      HIP_CHECK(hipMemcpyAsync(dmap[i], static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz,
+                               mn * esz, hipMemcpyHostToDevice, st));
   }
   run_device(c, j, dalm.data(), dmap.data(), alo, mlo, st, opcnt);
   if (j.type == SHARP_ALM2MAP) {
