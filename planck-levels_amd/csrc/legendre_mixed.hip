@@ -702,10 +702,12 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
                 const double rp1 = rec1p, rm1 = rec1m;
                 rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
                 const int k = 4 * g4 + 2 * h;
-                a1[k] = (float)(rp0 + rm0);
-                a2[k] = (float)(rm0 - rp0);
-                a1[k + 1] = (float)(rm1 - rp1);
-                a2[k + 1] = (float)(rp1 + rm1);
+                // combinations in FP32 of the FP32-rounded chains
+                const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
+                a1[k] = p0 + m0;
+                a2[k] = m0 - p0;
+                a1[k + 1] = m1 - p1;
+                a2[k + 1] = p1 + m1;
                 if (h == 0) {
                   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1248,10 +1250,19 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
         const double rp1 = rec1p, rm1 = rec1m;
         rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
-        mw[sidx(2 * j)] = (T)(rp0 + rm0);
-        mw[sidx(2 * j + 1)] = (T)(rp1 + rm1);
-        mx[sidx(2 * j)] = (T)(rm0 - rp0);
-        mx[sidx(2 * j + 1)] = (T)(rm1 - rp1);
+        if constexpr (sizeof(T) == 4) {
+          // combinations in FP32 of the FP32-rounded chains
+          const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
+          mw[sidx(2 * j)] = p0 + m0;
+          mw[sidx(2 * j + 1)] = p1 + m1;
+          mx[sidx(2 * j)] = m0 - p0;
+          mx[sidx(2 * j + 1)] = m1 - p1;
+        } else {
+          mw[sidx(2 * j)] = (T)(rp0 + rm0);
+          mw[sidx(2 * j + 1)] = (T)(rp1 + rm1);
+          mx[sidx(2 * j)] = (T)(rm0 - rp0);
+          mx[sidx(2 * j + 1)] = (T)(rm1 - rp1);
+        }
       }
       l += 32;
     } else {
