@@ -112,10 +112,12 @@ __device__ inline Bf3 split3(const float (&v)[8])
       mb[e] = ru;
       lb[e] = __float_as_uint(r2);
     }
-    // element 2p in the low half of word p, element 2p+1 in the high half
-    H[p] = (hb[1] & 0xffff0000u) | (hb[0] >> 16);
-    M[p] = (mb[1] & 0xffff0000u) | (mb[0] >> 16);
-    L[p] = (lb[1] & 0xffff0000u) | (lb[0] >> 16);
+    // element 2p in the low half of word p, element 2p+1 in the high half:
+    // (hi & 0xffff0000) | (lo >> 16) as one byte permute (bytes 2,3 of lo,
+    // then bytes 2,3 of hi)
+    H[p] = __builtin_amdgcn_perm(hb[1], hb[0], 0x07060302u);
+    M[p] = __builtin_amdgcn_perm(mb[1], mb[0], 0x07060302u);
+    L[p] = __builtin_amdgcn_perm(lb[1], lb[0], 0x07060302u);
   }
   return Bf3{__builtin_bit_cast(bf16x8, H), __builtin_bit_cast(bf16x8, M),
              __builtin_bit_cast(bf16x8, L)};
@@ -1373,7 +1375,15 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            tw[x][cb][v] += rsgn * __shfl_xor(tx[x][cb][v], 3);
+            if constexpr (sizeof(T) == 4) {
+              // lane i ^ 3 of the quad by a DPP quad permute [3,2,1,0]
+              const float t = tx[x][cb][v];
+              const float y = __int_as_float(
+                  __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0x1B, 0xF, 0xF, false));
+              tw[x][cb][v] = fmaf(rsgn, y, tw[x][cb][v]);
+            } else {
+              tw[x][cb][v] += rsgn * __shfl_xor(tx[x][cb][v], 3);
+            }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
