@@ -660,7 +660,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
           const int np = bs_pairs(lmax, st.mhi);
           const int czero = a.bs_ncol - 1, cbase = a.t0 * NALM * 2;
           const unsigned *px[NCB], *py[NCB];
-          unsigned sg1[NCB], sg2[NCB];
+          unsigned sgy[NCB];
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb) {
             const int col = cb * 16 + n, t = col >> 2, q = col & 3;
@@ -678,8 +678,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
             }
             px[cb] = bsm + (int64_t)3 * cx * np + 4 * kq;
             py[cb] = bsm + (int64_t)3 * cy * np + 4 * kq;
-            sg1[cb] = neg ? 0x80000000u : 0u;
-            sg2[cb] = neg ? 0x00008000u : 0u;
+            sgy[cb] = neg ? 0x80008000u : 0u;
           }
           while (l + 31 <= lmax) {
             const int l0 = l;
@@ -732,16 +731,22 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
               for (int r = 0; r < 4; ++r) { a1[8 * r + p] = t1[r]; a2[8 * r + p] = t2[r]; }
             }
             // accumulator 1: (X row 2d, Y row 2d+1) per word, accumulator 2:
-            // (Y row 2d, X row 2d+1), as split3 of the FP32 rows would give
+            // (Y row 2d, X row 2d+1), as split3 of the FP32 rows would give;
+            // both halves of the Y words take the sign first, then each word
+            // is one byte permute (low half of the first, high half of the
+            // second operand word)
             Bf3 B1[NCB], B2[NCB];
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
               u32x4 w1[3], w2[3];
 #pragma unroll
-              for (int q = 0; q < 3; ++q) {
-                w1[q] = ((PY[cb][q] & 0xffff0000u) | (PX[cb][q] & 0x0000ffffu)) ^ sg1[cb];
-                w2[q] = ((PX[cb][q] & 0xffff0000u) | (PY[cb][q] & 0x0000ffffu)) ^ sg2[cb];
-              }
+              for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  const unsigned x = PX[cb][q][e], y = PY[cb][q][e] ^ sgy[cb];
+                  w1[q][e] = __builtin_amdgcn_perm(y, x, 0x07060100u);
+                  w2[q][e] = __builtin_amdgcn_perm(x, y, 0x07060100u);
+                }
               B1[cb] = Bf3{__builtin_bit_cast(bf16x8, w1[0]), __builtin_bit_cast(bf16x8, w1[1]),
                            __builtin_bit_cast(bf16x8, w1[2])};
               B2[cb] = Bf3{__builtin_bit_cast(bf16x8, w2[0]), __builtin_bit_cast(bf16x8, w2[1]),
