@@ -133,9 +133,13 @@ struct YlmPlan {
   double *mfac = nullptr;    // spin 0
   double *prefac = nullptr;  // spin s
   int *fscale = nullptr;
+  // spin s, built for the first BF16 job: the table of lambda_l / s_l (all
+  // f2 = 1) and s_{m,l} at tri_ofs(m, lmax) + l - m
+  double *tab1 = nullptr, *lsc = nullptr;
   ~YlmPlan()
   {
     (void)hipFree(tab); (void)hipFree(mfac); (void)hipFree(prefac); (void)hipFree(fscale);
+    (void)hipFree(tab1); (void)hipFree(lsc);
   }
 };
 
@@ -487,6 +491,69 @@ static void host_normalize(double &val, int &scale, double xfmax)
     while (std::fabs(val) < xfmax * kFSmall) { val *= kFBig; --scale; }
 }
 
+// the spin-s recurrence coefficients (f0, f0 f1, f2) of entry l at
+// 3 (tab_ofs(m, lmax) + l - m), libsharp/sharp_ylmgen_c.c:150-175
+static std::vector<double> spin_table(int lmax, int mmax, int s)
+{
+  std::vector<double> tab(3 * (size_t)tab_ofs(mmax + 1, lmax), 0.);
+  std::vector<double> flm1(2 * lmax + 1), flm2(2 * lmax + 1), inv(lmax + 1);
+  inv[0] = 0;
+  for (int k = 1; k < lmax + 1; ++k) inv[k] = 1. / k;
+  for (int k = 0; k < 2 * lmax + 1; ++k) {
+    flm1[k] = std::sqrt(1. / (k + 1.));
+    flm2[k] = std::sqrt(k / (k + 1.));
+  }
+  for (int m = 0; m <= mmax; ++m) {
+    double *row = tab.data() + 3 * tab_ofs(m, lmax);
+    const int mhi = std::max(m, s);
+    for (int l = mhi; l < lmax; ++l) {
+      double t = flm1[l + m] * flm1[l - m] * flm1[l + s] * flm1[l - s];
+      const double lt = 2 * l + 1, l1 = l + 1;
+      double *e = row + 3 * (l + 1 - m);
+      e[0] = l1 * lt * t;
+      // stored as f0*f1: the kernels form (cth -+ f1) f0 as one FMA
+      e[1] = e[0] * (m * s * inv[l] * inv[l + 1]);
+      t = flm2[l + m] * flm2[l - m] * flm2[l + s] * flm2[l - s];
+      e[2] = t * l1 * inv[l];
+    }
+  }
+  return tab;
+}
+
+// The recurrence lambda_l = c_l lambda_{l-1} - f2_l lambda_{l-2}, c_l =
+// cth f0_l -+ g_l, rewritten for mu_l = lambda_l / s_l with s_l = f2_l s_{l-2}
+// and s = 1 at l = mhi, mhi+1 (the f2 of mhi+1 multiplies lambda_{mhi-1} = 0):
+// mu_l = c_l (s_{l-1} / s_l) mu_{l-1} - mu_{l-2}, one multiplication less per
+// step.  The BF16 spin-s kernels run it; the a_lm gather and scatter apply
+// s_{m,l}.  s stays within [0.01, 1] (it is a product of f2 <= 1 factors of
+// one parity).
+static void ensure_unit_table(YlmPlan &yp)
+{
+  if (yp.tab1 || yp.spin == 0) return;
+  const int lmax = yp.lmax, mmax = yp.mmax, s = yp.spin;
+  std::vector<double> tab = spin_table(lmax, mmax, s);
+  std::vector<double> lsc((size_t)tri_ofs(mmax + 1, lmax), 1.);
+  for (int m = 0; m <= mmax; ++m) {
+    double *row = tab.data() + 3 * (tab_ofs(m, lmax) - m);
+    double *sl = lsc.data() + tri_ofs(m, lmax) - m;
+    const int mhi = std::max(m, s);
+    for (int l = mhi + 2; l <= lmax; ++l) {
+      const double f2 = row[3 * l + 2];
+      if (!(f2 > 0.)) SHP_FAIL("spin recurrence coefficient f2 is not positive");
+      sl[l] = f2 * sl[l - 2];
+    }
+    for (int l = mhi + 1; l <= lmax; ++l) {
+      double *e = row + 3 * l;
+      const double r = sl[l - 1] / sl[l];
+      e[0] *= r;
+      e[1] *= r;
+      e[2] = 1.;
+    }
+  }
+  yp.tab1 = upload(tab);
+  yp.lsc = upload(lsc);
+}
+
 static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
 {
   for (auto &p : c.ylms)
@@ -520,28 +587,7 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
     yp->mfac = upload(mfac);
   } else {
     const int s = spin;
-    std::vector<double> tab(3 * nent, 0.);
-    std::vector<double> flm1(2 * lmax + 1), flm2(2 * lmax + 1), inv(lmax + 1);
-    inv[0] = 0;
-    for (int k = 1; k < lmax + 1; ++k) inv[k] = 1. / k;
-    for (int k = 0; k < 2 * lmax + 1; ++k) {
-      flm1[k] = std::sqrt(1. / (k + 1.));
-      flm2[k] = std::sqrt(k / (k + 1.));
-    }
-    for (int m = 0; m <= mmax; ++m) {
-      double *row = tab.data() + 3 * tab_ofs(m, lmax);
-      const int mhi = std::max(m, s);
-      for (int l = mhi; l < lmax; ++l) {
-        double t = flm1[l + m] * flm1[l - m] * flm1[l + s] * flm1[l - s];
-        const double lt = 2 * l + 1, l1 = l + 1;
-        double *e = row + 3 * (l + 1 - m);
-        e[0] = l1 * lt * t;
-        // stored as f0*f1: the kernels form (cth -+ f1) f0 as one FMA
-        e[1] = e[0] * (m * s * inv[l] * inv[l + 1]);
-        t = flm2[l + m] * flm2[l - m] * flm2[l + s] * flm2[l - s];
-        e[2] = t * l1 * inv[l];
-      }
-    }
+    std::vector<double> tab = spin_table(lmax, mmax, s);
     std::vector<double> fac(2 * lmax + 1);
     std::vector<int> facscale(2 * lmax + 1);
     fac[0] = 1;
@@ -867,6 +913,15 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   la.mlim = ml.d;
   la.ispair = gp.ispair;
   la.tab = yp.tab;
+  if ((j.flags & SHARP_HIP_MIXED_BF16X3) && j.spin > 0) {
+    // the BF16 spin-s kernels run the recurrence with f2 = 1; every kernel
+    // reading this table computes lambda_l / s_l, and the gather and scatter
+    // apply s_l
+    ensure_unit_table(yp);
+    la.tab = yp.tab1;
+    la.unit_tab = 1;
+    aa.lsc = yp.lsc;
+  }
   la.mfac = yp.mfac;
   la.prefac = yp.prefac;
   la.fscale = yp.fscale;

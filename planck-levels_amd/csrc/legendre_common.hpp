@@ -113,6 +113,16 @@ __device__ inline void rec_step(double &rxp, double &rxm, double ryp, double rym
   rxm = fma(fma(cth, f.f0, f.g), rym, -(f.f2 * rxm));
 }
 
+// rec_step on the table whose f2 are all 1: the recurrence of lambda_l / s_l
+// with s_l = f2_l s_{l-2} (s = 1 at the first two l), whose factor s_l the
+// a_lm gather and scatter apply instead (the BF16 spin-s kernels)
+__device__ inline void rec_step_unit(double &rxp, double &rxm, double ryp, double rym,
+                                     double cth, const Fx &f)
+{
+  rxp = fma(fma(cth, f.f0, -f.g), ryp, -rxp);
+  rxm = fma(fma(cth, f.f0, f.g), rym, -rxm);
+}
+
 __device__ inline bool rescale1(double &l1, double &l2, int &scale)
 {
   if (fabs(l2) > kFTol) {

@@ -699,9 +699,9 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
 #pragma unroll
               for (int h = 0; h < 2; ++h) {
                 const double rp0 = rec2p, rm0 = rec2m;
-                rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+                rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
                 const double rp1 = rec1p, rm1 = rec1m;
-                rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+                rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
                 const int k = 4 * g4 + 2 * h;
                 // combinations in FP32 of the FP32-rounded chains
                 const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
@@ -1254,9 +1254,16 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         const Fx f1{cq[6 * j], cq[6 * j + 1], cq[6 * j + 2]};
         const Fx f2{cq[6 * j + 3], cq[6 * j + 4], cq[6 * j + 5]};
         const double rp0 = rec2p, rm0 = rec2m;
-        rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
-        const double rp1 = rec1p, rm1 = rec1m;
-        rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
+        double rp1, rm1;
+        if constexpr (CT == 1) {
+          rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, f1);
+          rp1 = rec1p; rm1 = rec1m;
+          rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, f2);
+        } else {
+          rec_step(rec1p, rec1m, rec2p, rec2m, cth, f1);
+          rp1 = rec1p; rm1 = rec1m;
+          rec_step(rec2p, rec2m, rec1p, rec1m, cth, f2);
+        }
         if constexpr (sizeof(T) == 4) {
           // combinations in FP32 of the FP32-rounded chains
           const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
@@ -1532,6 +1539,7 @@ void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
     else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
   } else if (contr == kContrBf16x3) {
     if (!a.bsplit) SHP_FAIL("BF16 spin synthesis without the split a_lm");
+    if (!a.unit_tab) SHP_FAIL("BF16 spin synthesis without the unit-f2 table");
     // spin 0 keeps the FP32 matrix cores: with 16 columns the contraction is
     // too small a share for the splits to pay
     if (a.deriv1) {
@@ -1600,6 +1608,7 @@ void launch_manal(const LegArgs &a, int contr, hipStream_t st)
       if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2, 1>), grid, block, 0, st, a);
       else hipLaunchKernelGGL((k_manal_spin0<float, 1, 1>), grid, block, 0, st, a);
     } else {
+      if (!a.unit_tab) SHP_FAIL("BF16 spin analysis without the unit-f2 table");
       if (two) hipLaunchKernelGGL((k_manal_spin<float, 2, 1>), grid, block, 0, st, a);
       else hipLaunchKernelGGL((k_manal_spin<float, 1, 1>), grid, block, 0, st, a);
     }

@@ -265,6 +265,7 @@ __device__ inline double2 gather_value(const AlmArgs &a, int m, int l, int comp)
   const void *p = a.alms.p[comp];
   const double re = ld<T>(p, idx), im = (realonly || m == 0) ? 0. : ld<T>(p, idx + 1);
   double f = a.norm_l[l];
+  if (a.lsc) f *= a.lsc[tri_ofs(m, a.lmax) + l - m];
   if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
   return make_double2(re * f, im * f);
 }
@@ -400,6 +401,7 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
   const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
   void *p = a.alms.p[comp];
   double f = a.norm_l[l];
+  if (a.lsc) f *= a.lsc[row];
   if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrtHalf;
   const bool add = (a.flags & SHARP_ADD) != 0;
   st<T>(p, idx, (add ? ld<T>(p, idx) : 0.) + x.x * f);
