@@ -133,8 +133,8 @@ struct YlmPlan {
   double *mfac = nullptr;    // spin 0
   double *prefac = nullptr;  // spin s
   int *fscale = nullptr;
-  // spin s, built for the first BF16 job: the table of lambda_l / s_l (all
-  // f2 = 1) and s_{m,l} at tri_ofs(m, lmax) + l - m
+  // built for the first mixed spin-0 or BF16 spin-s job: the table of
+  // lambda_l / s_l (all b_l or f2_l = 1) and s_{m,l} at tri_ofs(m, lmax) + l - m
   double *tab1 = nullptr, *lsc = nullptr;
   ~YlmPlan()
   {
@@ -520,6 +520,50 @@ static std::vector<double> spin_table(int lmax, int mmax, int s)
   return tab;
 }
 
+// the spin-0 recurrence coefficients (a_l, b_l) of entry l at
+// 2 (tab_ofs(m, lmax) + l - m), libsharp/sharp_ylmgen_c.c:88-110
+static std::vector<double> spin0_table(int lmax, int mmax)
+{
+  std::vector<double> tab(2 * (size_t)tab_ofs(mmax + 1, lmax), 0.), root(2 * lmax + 5),
+      iroot(2 * lmax + 5);
+  for (int k = 0; k < 2 * lmax + 5; ++k) {
+    root[k] = std::sqrt((double)k);
+    iroot[k] = (k == 0) ? 0. : 1. / root[k];
+  }
+  for (int m = 0; m <= mmax; ++m) {
+    double *row = tab.data() + 2 * tab_ofs(m, lmax);
+    row[0] = root[2 * m + 3];
+    row[1] = 0.;
+    for (int l = m + 1; l <= lmax; ++l) {
+      const double tmp = root[2 * l + 3] * iroot[l + 1 + m] * iroot[l + 1 - m];
+      row[2 * (l - m)] = tmp * root[2 * l + 1];
+      row[2 * (l - m) + 1] = tmp * root[l + m] * root[l - m] * iroot[2 * l - 1];
+    }
+  }
+  return tab;
+}
+
+// Spin 0: entry l gives lambda_{l+1} = a_l cth lambda_l - b_l lambda_{l-1};
+// with s_m = s_{m+1} = 1 and s_{l+1} = b_l s_{l-1}, mu_l = lambda_l / s_l obeys
+// mu_{l+1} = a_l (s_l / s_{l+1}) cth mu_l - mu_{l-1} (s within [0.1, 1.2]).
+// The mixed spin-0 kernels run it.
+static void unit_table0(const YlmPlan &yp, std::vector<double> &tab, std::vector<double> &lsc)
+{
+  const int lmax = yp.lmax, mmax = yp.mmax;
+  tab = spin0_table(lmax, mmax);
+  for (int m = 0; m <= mmax; ++m) {
+    double *row = tab.data() + 2 * (tab_ofs(m, lmax) - m);
+    double *sl = lsc.data() + tri_ofs(m, lmax) - m;
+    for (int l = m + 2; l <= lmax; ++l) {
+      const double b = row[2 * (l - 1) + 1];
+      if (!(b > 0.)) SHP_FAIL("spin-0 recurrence coefficient b is not positive");
+      sl[l] = b * sl[l - 2];
+    }
+    for (int l = m; l < lmax; ++l) row[2 * l] *= sl[l] / sl[l + 1];
+    for (int l = m; l <= lmax; ++l) row[2 * l + 1] = 1.;
+  }
+}
+
 // The recurrence lambda_l = c_l lambda_{l-1} - f2_l lambda_{l-2}, c_l =
 // cth f0_l -+ g_l, rewritten for mu_l = lambda_l / s_l with s_l = f2_l s_{l-2}
 // and s = 1 at l = mhi, mhi+1 (the f2 of mhi+1 multiplies lambda_{mhi-1} = 0):
@@ -529,10 +573,16 @@ static std::vector<double> spin_table(int lmax, int mmax, int s)
 // one parity).
 static void ensure_unit_table(YlmPlan &yp)
 {
-  if (yp.tab1 || yp.spin == 0) return;
+  if (yp.tab1) return;
   const int lmax = yp.lmax, mmax = yp.mmax, s = yp.spin;
-  std::vector<double> tab = spin_table(lmax, mmax, s);
-  std::vector<double> lsc((size_t)tri_ofs(mmax + 1, lmax), 1.);
+  std::vector<double> tab, lsc((size_t)tri_ofs(mmax + 1, lmax), 1.);
+  if (s == 0) {
+    unit_table0(yp, tab, lsc);
+    yp.tab1 = upload(tab);
+    yp.lsc = upload(lsc);
+    return;
+  }
+  tab = spin_table(lmax, mmax, s);
   for (int m = 0; m <= mmax; ++m) {
     double *row = tab.data() + 3 * (tab_ofs(m, lmax) - m);
     double *sl = lsc.data() + tri_ofs(m, lmax) - m;
@@ -562,23 +612,8 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
   yp->lmax = lmax;
   yp->mmax = mmax;
   yp->spin = spin;
-  const size_t nent = (size_t)tab_ofs(mmax + 1, lmax);
   if (spin == 0) {
-    std::vector<double> tab(2 * nent, 0.), root(2 * lmax + 5), iroot(2 * lmax + 5);
-    for (int k = 0; k < 2 * lmax + 5; ++k) {
-      root[k] = std::sqrt((double)k);
-      iroot[k] = (k == 0) ? 0. : 1. / root[k];
-    }
-    for (int m = 0; m <= mmax; ++m) {
-      double *row = tab.data() + 2 * tab_ofs(m, lmax);
-      row[0] = root[2 * m + 3];
-      row[1] = 0.;
-      for (int l = m + 1; l <= lmax; ++l) {
-        const double tmp = root[2 * l + 3] * iroot[l + 1 + m] * iroot[l + 1 - m];
-        row[2 * (l - m)] = tmp * root[2 * l + 1];
-        row[2 * (l - m) + 1] = tmp * root[l + m] * root[l - m] * iroot[2 * l - 1];
-      }
-    }
+    std::vector<double> tab = spin0_table(lmax, mmax);
     std::vector<double> mfac(mmax + 1);
     mfac[0] = 0.2820947917738781434740397257803862929220;   // 1/sqrt(4 pi)
     for (int m = 1; m <= mmax; ++m)
@@ -913,10 +948,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   la.mlim = ml.d;
   la.ispair = gp.ispair;
   la.tab = yp.tab;
-  if ((j.flags & SHARP_HIP_MIXED_BF16X3) && j.spin > 0) {
-    // the BF16 spin-s kernels run the recurrence with f2 = 1; every kernel
-    // reading this table computes lambda_l / s_l, and the gather and scatter
-    // apply s_l
+  if ((j.flags & SHARP_HIP_MIXED_BF16X3) || (is_mixed(j.flags) && j.spin == 0)) {
+    // the mixed spin-0 and BF16 spin-s kernels run the recurrence with a unit
+    // lambda_{l-2} factor; every kernel reading this table computes
+    // lambda_l / s_l, and the gather and scatter apply s_l
     ensure_unit_table(yp);
     la.tab = yp.tab1;
     la.unit_tab = 1;

@@ -409,9 +409,10 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
             for (int q = 0; q < 4; ++q) {
               const double2 r0 = rc[2 * q], r1 = rc[2 * q + 1];
               v1[q] = (float)lam2;
-              lam1 = (q == 0 ? t00 * r00x : (cth * lam2) * r0.x) - lam1 * r0.y;
+              // unit-b table (every b_l = 1): lambda_l / s_l
+              lam1 = (q == 0 ? t00 * r00x : (cth * lam2) * r0.x) - lam1;
               v2[q] = (float)lam1;
-              lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+              lam2 = (cth * lam1) * r1.x - lam2;
               if (q == 1) mfmas(pv1, pv2, pb1, pb2);
             }
             row_transpose4(v1);
@@ -1021,9 +1022,18 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const double o0 = lam2;
-        lam1 = (cth * lam2) * cq[4 * q] - lam1 * cq[4 * q + 1];
+        if constexpr (sizeof(T) == 4) {
+          // unit-b table (every b_l = 1): lambda_l / s_l
+          lam1 = (cth * lam2) * cq[4 * q] - lam1;
+        } else {
+          lam1 = (cth * lam2) * cq[4 * q] - lam1 * cq[4 * q + 1];
+        }
         const double o1 = lam1;
-        lam2 = (cth * lam1) * cq[4 * q + 2] - lam2 * cq[4 * q + 3];
+        if constexpr (sizeof(T) == 4) {
+          lam2 = (cth * lam1) * cq[4 * q + 2] - lam2;
+        } else {
+          lam2 = (cth * lam1) * cq[4 * q + 2] - lam2 * cq[4 * q + 3];
+        }
         mine[sidx(2 * q)] = (T)o0;
         mine[sidx(2 * q + 1)] = (T)o1;
       }
@@ -1535,6 +1545,7 @@ void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
   if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
   const bool two = cols > 16;
   if (a.spin == 0) {
+    if (!a.unit_tab) SHP_FAIL("mixed spin-0 synthesis without the unit-b table");
     if (two) hipLaunchKernelGGL(k_msynth_spin0<2>, grid, block, 0, st, a, nsb);
     else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
   } else if (contr == kContrBf16x3) {
@@ -1603,6 +1614,8 @@ void launch_manal(const LegArgs &a, int contr, hipStream_t st)
   if (f64) {
     if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1, 0>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((k_manal_spin<double, 1, 0>), grid, block, 0, st, a);
+  } else if (a.spin == 0 && !a.unit_tab) {
+    SHP_FAIL("mixed spin-0 analysis without the unit-b table");
   } else if (bf) {
     if (a.spin == 0) {
       if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2, 1>), grid, block, 0, st, a);
