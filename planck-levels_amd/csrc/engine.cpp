@@ -133,7 +133,7 @@ struct YlmPlan {
   double *mfac = nullptr;    // spin 0
   double *prefac = nullptr;  // spin s
   int *fscale = nullptr;
-  // built for the first mixed spin-0 or BF16 spin-s job: the table of
+  // built for the first mixed-precision job: the table of
   // lambda_l / s_l (all b_l or f2_l = 1) and s_{m,l} at tri_ofs(m, lmax) + l - m
   double *tab1 = nullptr, *lsc = nullptr;
   ~YlmPlan()
@@ -546,7 +546,7 @@ static std::vector<double> spin0_table(int lmax, int mmax)
 // Spin 0: entry l gives lambda_{l+1} = a_l cth lambda_l - b_l lambda_{l-1};
 // with s_m = s_{m+1} = 1 and s_{l+1} = b_l s_{l-1}, mu_l = lambda_l / s_l obeys
 // mu_{l+1} = a_l (s_l / s_{l+1}) cth mu_l - mu_{l-1} (s within [0.1, 1.2]).
-// The mixed spin-0 kernels run it.
+// The mixed-precision kernels run it.
 static void unit_table0(const YlmPlan &yp, std::vector<double> &tab, std::vector<double> &lsc)
 {
   const int lmax = yp.lmax, mmax = yp.mmax;
@@ -568,7 +568,7 @@ static void unit_table0(const YlmPlan &yp, std::vector<double> &tab, std::vector
 // cth f0_l -+ g_l, rewritten for mu_l = lambda_l / s_l with s_l = f2_l s_{l-2}
 // and s = 1 at l = mhi, mhi+1 (the f2 of mhi+1 multiplies lambda_{mhi-1} = 0):
 // mu_l = c_l (s_{l-1} / s_l) mu_{l-1} - mu_{l-2}, one multiplication less per
-// step.  The BF16 spin-s kernels run it; the a_lm gather and scatter apply
+// step.  The mixed-precision kernels run it; the a_lm gather and scatter apply
 // s_{m,l}.  s stays within [0.01, 1] (it is a product of f2 <= 1 factors of
 // one parity).
 static void ensure_unit_table(YlmPlan &yp)
@@ -948,10 +948,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   la.mlim = ml.d;
   la.ispair = gp.ispair;
   la.tab = yp.tab;
-  if ((j.flags & SHARP_HIP_MIXED_BF16X3) || (is_mixed(j.flags) && j.spin == 0)) {
-    // the mixed spin-0 and BF16 spin-s kernels run the recurrence with a unit
-    // lambda_{l-2} factor; every kernel reading this table computes
-    // lambda_l / s_l, and the gather and scatter apply s_l
+  if (is_mixed(j.flags)) {
+    // the mixed kernels run the recurrence with a unit lambda_{l-2} factor;
+    // every kernel reading this table computes lambda_l / s_l, and the gather
+    // and scatter apply s_l
     ensure_unit_table(yp);
     la.tab = yp.tab1;
     la.unit_tab = 1;

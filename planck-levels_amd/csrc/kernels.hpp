@@ -20,7 +20,7 @@ struct LegArgs {
   const double *tab;    // recurrence coefficients, rows at tab_ofs(m,lmax)
   int unit_tab;         // 1: tab holds the coefficients of lambda_l / s_l whose
                         // lambda_{l-2} factor is 1 (spin 0: b_l; spin s: f2);
-                        // the mixed spin-0 and BF16 spin-s kernels need it
+                        // the mixed-precision kernels need it
   const double *mfac;   // spin 0: lambda_mm prefactors [mmax+1]
   const double *prefac; // spin s: start-value prefactors [mmax+1]
   const int *fscale;    // spin s: their scales [mmax+1]
@@ -159,7 +159,7 @@ struct AlmArgs {
   int64_t alm_base;     // subtracted from every a_lm index (real units)
   const int64_t *mvstart_by_m;  // [mmax+1]
   const double *norm_l; // [lmax+1]
-  // with a unit table (mixed spin 0, BF16 spin s): s_{m,l} at
+  // with a unit table (the mixed-precision forms): s_{m,l} at
   // tri_ofs(m,lmax)+l-m, applied beside norm_l; else nullptr
   const double *lsc;
   double2 *almtmp;

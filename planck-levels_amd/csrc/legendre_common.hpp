@@ -115,7 +115,7 @@ __device__ inline void rec_step(double &rxp, double &rxm, double ryp, double rym
 
 // rec_step on the table whose f2 are all 1: the recurrence of lambda_l / s_l
 // with s_l = f2_l s_{l-2} (s = 1 at the first two l), whose factor s_l the
-// a_lm gather and scatter apply instead (the BF16 spin-s kernels)
+// a_lm gather and scatter apply instead (the mixed-precision kernels)
 __device__ inline void rec_step_unit(double &rxp, double &rxm, double ryp, double rym,
                                      double cth, const Fx &f)
 {

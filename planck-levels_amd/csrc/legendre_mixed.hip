@@ -797,10 +797,11 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
             for (int q = 0; q < 4; ++q) fn[q] = fxu(lb + 5 + q);
             __builtin_amdgcn_sched_barrier(0);
             {
+              // unit-f2 table: lambda_l / s_l
               const double rp0 = rec2p, rm0 = rec2m;
-              rec1p = fma(c0m, rec2p, -(fc[0].f2 * rec1p));
-              rec1m = fma(c0p, rec2m, -(fc[0].f2 * rec1m));
-              rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[1]);
+              rec1p = fma(c0m, rec2p, -rec1p);
+              rec1m = fma(c0p, rec2m, -rec1m);
+              rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, fc[1]);
               v1[0] = (float)(rp0 + rm0);
               v2[0] = (float)(rm0 - rp0);
               v2[1] = (float)(rec1p + rec1m);
@@ -808,8 +809,8 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
             }
             {
               const double rp0 = rec2p, rm0 = rec2m;
-              rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2]);
-              rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[3]);
+              rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, fc[2]);
+              rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, fc[3]);
               v1[2] = (float)(rp0 + rm0);
               v2[2] = (float)(rm0 - rp0);
               v2[3] = (float)(rec1p + rec1m);
@@ -1265,7 +1266,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         const Fx f2{cq[6 * j + 3], cq[6 * j + 4], cq[6 * j + 5]};
         const double rp0 = rec2p, rm0 = rec2m;
         double rp1, rm1;
-        if constexpr (CT == 1) {
+        if constexpr (sizeof(T) == 4) {
+          // unit-f2 table (the mixed forms): lambda_l / s_l
           rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, f1);
           rp1 = rec1p; rm1 = rec1m;
           rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, f2);
@@ -1544,13 +1546,12 @@ void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
   if (a.ntl < 1 || cols > 32) SHP_FAIL("bad transform group");
   const bool two = cols > 16;
+  if (!a.unit_tab) SHP_FAIL("mixed synthesis without the unit-coefficient table");
   if (a.spin == 0) {
-    if (!a.unit_tab) SHP_FAIL("mixed spin-0 synthesis without the unit-b table");
     if (two) hipLaunchKernelGGL(k_msynth_spin0<2>, grid, block, 0, st, a, nsb);
     else hipLaunchKernelGGL(k_msynth_spin0<1>, grid, block, 0, st, a, nsb);
   } else if (contr == kContrBf16x3) {
     if (!a.bsplit) SHP_FAIL("BF16 spin synthesis without the split a_lm");
-    if (!a.unit_tab) SHP_FAIL("BF16 spin synthesis without the unit-f2 table");
     // spin 0 keeps the FP32 matrix cores: with 16 columns the contraction is
     // too small a share for the splits to pay
     if (a.deriv1) {
@@ -1614,14 +1615,13 @@ void launch_manal(const LegArgs &a, int contr, hipStream_t st)
   if (f64) {
     if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1, 0>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((k_manal_spin<double, 1, 0>), grid, block, 0, st, a);
-  } else if (a.spin == 0 && !a.unit_tab) {
-    SHP_FAIL("mixed spin-0 analysis without the unit-b table");
+  } else if (!a.unit_tab) {
+    SHP_FAIL("mixed analysis without the unit-coefficient table");
   } else if (bf) {
     if (a.spin == 0) {
       if (two) hipLaunchKernelGGL((k_manal_spin0<float, 2, 1>), grid, block, 0, st, a);
       else hipLaunchKernelGGL((k_manal_spin0<float, 1, 1>), grid, block, 0, st, a);
     } else {
-      if (!a.unit_tab) SHP_FAIL("BF16 spin analysis without the unit-f2 table");
       if (two) hipLaunchKernelGGL((k_manal_spin<float, 2, 1>), grid, block, 0, st, a);
       else hipLaunchKernelGGL((k_manal_spin<float, 1, 1>), grid, block, 0, st, a);
     }
