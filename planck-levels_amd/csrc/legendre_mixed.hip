@@ -1403,7 +1403,7 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
               // lane i ^ 3 of the quad by a DPP quad permute [3,2,1,0]
               const float t = tx[x][cb][v];
               const float y = __int_as_float(
-                  __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0x1B, 0xF, 0xF, false));
+                  __builtin_amdgcn_mov_dpp(__float_as_int(t), 0x1B, 0xF, 0xF, false));
               tw[x][cb][v] = fmaf(rsgn, y, tw[x][cb][v]);
             } else {
               tw[x][cb][v] += rsgn * __shfl_xor(tx[x][cb][v], 3);
