@@ -84,6 +84,31 @@ template <typename T> __device__ inline void st(void *p, int64_t i, double v)
 {
   static_cast<T *>(p)[i] = (T)v;
 }
+// elements i, i+1 as one vector access: needs unit stride and p + i aligned
+// to two elements for every even i (checked once per ring by pair_aligned)
+template <typename T> __device__ inline bool pair_aligned(const void *p, int64_t ofs,
+                                                          int64_t stride)
+{
+  return stride == 1 && (ofs & 1) == 0 &&
+         (reinterpret_cast<uintptr_t>(p) & (2 * sizeof(T) - 1)) == 0;
+}
+template <typename T> __device__ inline double2 ld_pair(const void *p, int64_t i)
+{
+  if constexpr (sizeof(T) == 8) {
+    return *reinterpret_cast<const double2 *>(static_cast<const double *>(p) + i);
+  } else {
+    const float2 v = *reinterpret_cast<const float2 *>(static_cast<const float *>(p) + i);
+    return make_double2(v.x, v.y);
+  }
+}
+template <typename T> __device__ inline void st_pair(void *p, int64_t i, double2 v)
+{
+  if constexpr (sizeof(T) == 8) {
+    *reinterpret_cast<double2 *>(static_cast<double *>(p) + i) = v;
+  } else {
+    *reinterpret_cast<float2 *>(static_cast<float *>(p) + i) = make_float2((float)v.x, (float)v.y);
+  }
+}
 
 // --- small DFTs: X_q = sum_r v_r e^{sign 2 pi i r q / R} ---------------------
 template <int R> struct Dft;
@@ -608,13 +633,19 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
     // 4. store
     void *mp = a.maps.p[comp];
     const bool add = (a.flags & SHARP_ADD) != 0;
+    const bool pairs = even && pair_aligned<T>(mp, r.ofs - a.map_base, r.stride);
     for (int j = tid; j < N; j += NT) {
       const double2 z = buf[j];
       if (even) {
         const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
         const int64_t i1 = i0 + r.stride;
-        st<T>(mp, i0, (add ? ld<T>(mp, i0) : 0.) + z.x);
-        st<T>(mp, i1, (add ? ld<T>(mp, i1) : 0.) + z.y);
+        if (pairs) {
+          const double2 o = add ? ld_pair<T>(mp, i0) : make_double2(0., 0.);
+          st_pair<T>(mp, i0, make_double2(o.x + z.x, o.y + z.y));
+        } else {
+          st<T>(mp, i0, (add ? ld<T>(mp, i0) : 0.) + z.x);
+          st<T>(mp, i1, (add ? ld<T>(mp, i1) : 0.) + z.y);
+        }
       } else {
         const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
         st<T>(mp, i0, (add ? ld<T>(mp, i0) : 0.) + z.x);
@@ -656,11 +687,14 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
     const void *mp = a.maps.p[comp];
+    // pixel pairs as one vector access: unit stride, pair start aligned
+    const bool pairs = even && pair_aligned<T>(mp, r.ofs - a.map_base, r.stride);
     // 1. load: even n -> z_j = f_2j + i f_2j+1, odd n -> z_j = f_j
     batched<NT, (P2 == 2) ? 4 : 8>(0, N,
         [&](int j) {
           if (even) {
             const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
+            if (pairs) return ld_pair<T>(mp, i0);
             return make_double2(ld<T>(mp, i0), ld<T>(mp, i0 + r.stride));
           }
           const int64_t i0 = r.ofs + (int64_t)j * r.stride - a.map_base;
