@@ -449,7 +449,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU form of this path "
-              "is not built (see DESIGN.md); use --gpus 1", file=sys.stderr)
+              "(realisations split over GPUs, or an m-sharded transform with one "
+              "all-to-all of the phases per direction) is not built; use --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
