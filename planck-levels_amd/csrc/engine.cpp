@@ -1796,7 +1796,8 @@ void sharp_hip_prepare(const sharp_geom_info *geom_info,
   const int mmax = alm_info->nm - 1;
   GeomPlan &gp = get_geom(c, geom_info);
   get_mlim(gp, alm_info->lmax, spin);
-  get_ylm(c, alm_info->lmax, mmax, spin);
+  // libsharp's table (FP64 form) and the unit-coefficient one (mixed forms)
+  ensure_unit_table(get_ylm(c, alm_info->lmax, mmax, spin));
   get_norm(c, alm_info->lmax, spin, 0);
   get_almplan(c, alm_info, mmax);
 }
