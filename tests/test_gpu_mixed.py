@@ -57,6 +57,17 @@ def test_several_launches(product, oracle, gpu, spin, ntrans, form):
         compare_mixed(product, oracle, iso("gauss", 17, 33), tri(16), jt, spin, ntrans, form=form)
 
 
+@pytest.mark.parametrize("spin", [0, 1, 2, 3])
+def test_band_limit_near_spin(product, oracle, gpu, spin, form):
+    """lmax = spin .. spin+3: rows whose recurrence has no step, one step (the
+    first, whose lambda_{l-2} term is zero), or the first steps with a scale
+    factor s_{m,l} != 1 in the unit-coefficient tables of the mixed forms."""
+    for lmax in range(spin, spin + 4):
+        for jt in (O.ALM2MAP, O.MAP2ALM):
+            compare_mixed(product, oracle, iso("gauss", 9, 17), tri(lmax), jt, spin, 2,
+                          form=form)
+
+
 def test_deriv1(product, oracle, gpu, form):
     compare_mixed(product, oracle, healpix(8), tri(20), O.DERIV1, 1, 2, form=form)
 
