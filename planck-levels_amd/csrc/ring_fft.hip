@@ -532,7 +532,57 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
   for (int comp = c0; comp < c1; ++comp) {
     // 1. fold: C_k = sum over m = k (mod n) of c_m + conj(sum over m = -k
     //    (mod n), m > 0, of c_m)
+    bool packed = false;
     if constexpr (!STAGED) {
+      if (even) {
+        // n > mmax, even n: fold and pack in one pass.  The thread of k in
+        // [0, N/2] forms the bins C_k and C_{N-k} (each at most one phase
+        // and one conjugated alias) and writes the packed Z_k and Z_{N-k}
+        // (step 2 below), with no LDS round trip between the two steps.
+        // Rotations: e^{i(n-k)phi0} = rn conj(rk), e^{i(N-k)phi0} = rN
+        // conj(rk), e^{i(N+k)phi0} = rN rk.
+        const double2 rN = norot ? one : expi_small((double)N * r.phi0);
+        const double tsc = 2.0 / n;
+        auto phase_at = [&](int m) {
+          return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
+        };
+        const double2 zero = make_double2(0., 0.);
+        for (int base = tid; 2 * base <= N; base += 2 * NT) {
+          double2 va[2], vb[2], vc[2], vd[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int k = base + u * NT, j = N - k, mb = (k == 0) ? n : n - k;
+            const bool ok = 2 * k <= N;
+            va[u] = (ok && k <= mmax) ? phase_at(k) : zero;
+            vb[u] = (ok && mb <= mmax) ? phase_at(mb) : zero;
+            vc[u] = (ok && j <= mmax) ? phase_at(j) : zero;
+            vd[u] = (ok && N + k <= mmax) ? phase_at(N + k) : zero;
+          }
+          double2 rk = norot ? one : expi_small((double)base * r.phi0);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int k = base + u * NT, j = N - k, mb = (k == 0) ? n : n - k;
+            if (2 * k <= N) {
+              double2 ck = zero, cj = zero;
+              if (k <= mmax) ck = cval(k, va[u], rk);
+              if (mb <= mmax) ck = cadd(ck, conj(cval(mb, vb[u], (k == 0) ? rn : cmul(rn, conj(rk)))));
+              if (j <= mmax) cj = cval(j, vc[u], cmul(rN, conj(rk)));
+              if (N + k <= mmax) cj = cadd(cj, conj(cval(N + k, vd[u], cmul(rN, rk))));
+              const double2 tw = expipi((double)k * tsc);
+              const double2 e = cadd(ck, conj(cj));
+              const double2 o = cmul(csub(ck, conj(cj)), tw);
+              buf[k] = make_double2(e.x - o.y, e.y + o.x);
+              if (j < N && j != k) {
+                const double2 e2 = cadd(cj, conj(ck));
+                const double2 o2 = cmul(csub(cj, conj(ck)), make_double2(-tw.x, tw.y));
+                buf[j] = make_double2(e2.x - o2.y, e2.y + o2.x);
+              }
+            }
+            rk = cmul(rk, rstep);
+          }
+        }
+        packed = true;
+      } else
       // n > mmax: at most c_k and conj(c_{n-k}) per bin; 4 bins (8 loads)
       // in flight per thread
       for (int base = tid; base < nbins; base += 4 * NT) {
@@ -611,7 +661,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
     }
   // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
     //    (e^{2 pi i (N-k)/n} = -conj(e^{2 pi i k/n}) since n = 2N)
-    if (even) {
+    if (even && !packed) {
       const double tsc = 2.0 / n;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
