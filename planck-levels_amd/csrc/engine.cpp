@@ -569,8 +569,9 @@ static void unit_table0(const YlmPlan &yp, std::vector<double> &tab, std::vector
 // and s = 1 at l = mhi, mhi+1 (the f2 of mhi+1 multiplies lambda_{mhi-1} = 0):
 // mu_l = c_l (s_{l-1} / s_l) mu_{l-1} - mu_{l-2}, one multiplication less per
 // step.  The mixed-precision kernels run it; the a_lm gather and scatter apply
-// s_{m,l}.  s stays within [0.01, 1] (it is a product of f2 <= 1 factors of
-// one parity).
+// s_{m,l}.  s lies in (0, 1]: it is a product of factors f2_l <= l/(l+1) of
+// one parity ((l^2-m^2)/((l+1)^2-m^2) <= l^2/(l+1)^2, likewise for s); its
+// minimum is 0.017 at lmax 4096 and 0.008 at lmax 16384.
 static void ensure_unit_table(YlmPlan &yp)
 {
   if (yp.tab1) return;
