@@ -313,7 +313,8 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
     }
   }
   if (r0 >= np) return;
-  const int nr = min(64, np - r0), ncol = min(32, a.bs_ncol - 2 * c0);
+  // real columns of this chunk; the last chunk also writes the zero column
+  const int nr = min(64, np - r0), ncol = min(32, ncf - 2 * c0);
   const int lb = o0 + 2 * r0;
   for (int e = threadIdx.x; e < 128 * 16; e += 256) {
     const int i = e & 127, cc = e >> 7, l = lb + i, comp = c0 + cc;
@@ -343,14 +344,22 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
     w[(int64_t)(3 * (2 * c0 + cc) + q) * np + r0 + p] =
         (piece(tile[2 * p + 1][cc]) << 16) | piece(tile[2 * p][cc]);
   }
+  if (2 * c0 + 32 >= ncf) {
+    const int zc = a.bs_ncol - 1;
+    for (int e = threadIdx.x; e < 3 * 64; e += 256) {
+      const int p = e & 63, q = e >> 6;
+      if (p < nr) w[(int64_t)(3 * zc + q) * np + r0 + p] = 0u;
+    }
+  }
 }
 
 void launch_alm_gather_split(const AlmArgs &a, hipStream_t s)
 {
   // tiles from the longest m (pairs from max(m, spin)); chunks of 16
-  // components, the last one holding the zero column
+  // components, the last one also writing the zero column
   const int ntiles = std::max(1, (bs_pairs(a.lmax, a.spin) + 63) / 64);
-  const int nchunks = (a.bs_ncol + 31) / 32;
+  const int nchunks = std::max(1, (2 * a.ncomp + 31) / 32);
+  if (a.bs_ncol != 2 * a.ncomp + 1) SHP_FAIL("BF16 a_lm split: bad column count");
   const dim3 grid(ntiles * nchunks, a.mmax + 1);
   if (a.flags & SHARP_DP)
     hipLaunchKernelGGL(k_alm_gather_split<double>, grid, dim3(256), 0, s, a, ntiles);
