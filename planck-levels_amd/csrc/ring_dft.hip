@@ -371,15 +371,29 @@ void launch_alm_gather_split(const AlmArgs &a, hipStream_t s)
 // components [c0, c0+npc); the values come from almtmp, or (part != nullptr)
 // from the partial sums of the ring-pair groups that reach m, added in FP64
 // in group order
+// The m of canonical row `row` (rows of m at tri_ofs(m, lmax) ..): the
+// root of m (2 lmax + 3 - m) / 2 = row, then an exact integer correction
+__device__ inline int row_m(int64_t row, int lmax)
+{
+  const double b = 2.0 * lmax + 3.0;
+  int m = (int)((b - sqrt(fmax(b * b - 8.0 * (double)row, 0.))) * 0.5);
+  while (m > 0 && tri_ofs(m, lmax) > row) --m;
+  while (tri_ofs(m + 1, lmax) <= row) ++m;
+  return m;
+}
+
+// one thread per (row, component) of the triangle: a flat grid with no idle
+// workgroups
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
 {
-  const int m = blockIdx.y;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int npc = a.npc;
-  if (e >= (int64_t)(a.lmax - m + 1) * npc) return;
-  const int l = m + (int)(e / npc), cl = (int)(e % npc), comp = a.c0 + cl;
-  const int64_t row = tri_ofs(m, a.lmax) + l - m;
+  if (e >= tri_ofs(a.mmax + 1, a.lmax) * npc) return;
+  const int64_t row = e / npc;
+  const int cl = (int)(e - row * npc), comp = a.c0 + cl;
+  const int m = row_m(row, a.lmax);
+  const int l = m + (int)(row - tri_ofs(m, a.lmax));
   double2 x;
   if (a.part) {
     const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * a.part_ld;
@@ -453,8 +467,8 @@ void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_
 
 void launch_alm_scatter(const AlmArgs &a, hipStream_t s)
 {
-  const int64_t per = (int64_t)(a.lmax + 1) * a.npc;
-  const dim3 grid((unsigned)((per + 255) / 256), a.mmax + 1);
+  const int64_t tot = tri_ofs(a.mmax + 1, a.lmax) * a.npc;
+  const dim3 grid((unsigned)((tot + 255) / 256));
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_scatter<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_scatter<float>, grid, dim3(256), 0, s, a);
   check_launch("alm_scatter");
