@@ -270,16 +270,30 @@ __device__ inline double2 gather_value(const AlmArgs &a, int m, int l, int comp)
   return make_double2(re * f, im * f);
 }
 
+// The m of canonical row `row` (rows of m at tri_ofs(m, lmax) ..): the
+// root of m (2 lmax + 3 - m) / 2 = row, then an exact integer correction
+__device__ inline int row_m(int64_t row, int lmax)
+{
+  const double b = 2.0 * lmax + 3.0;
+  int m = (int)((b - sqrt(fmax(b * b - 8.0 * (double)row, 0.))) * 0.5);
+  while (m > 0 && tri_ofs(m, lmax) > row) --m;
+  while (tri_ofs(m + 1, lmax) <= row) ++m;
+  return m;
+}
+
+// one thread per (row, component) of the triangle, as the scatter
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
 {
-  const int m = blockIdx.y;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int nc = a.ncomp;
-  if (e >= (int64_t)(a.lmax - m + 1) * nc) return;
-  const int l = m + (int)(e / nc), comp = (int)(e % nc);
+  if (e >= tri_ofs(a.mmax + 1, a.lmax) * nc) return;
+  const int64_t row = e / nc;
+  const int comp = (int)(e - row * nc);
+  const int m = row_m(row, a.lmax);
+  const int l = m + (int)(row - tri_ofs(m, a.lmax));
   const double2 v = gather_value<T>(a, m, l, comp);
-  const int64_t o = (int64_t)tri_ofs(m, a.lmax) * nc + e;
+  const int64_t o = e;
   if (a.tmp_f32)
     reinterpret_cast<float2 *>(a.almtmp)[o] = make_float2((float)v.x, (float)v.y);
   else
@@ -371,17 +385,6 @@ void launch_alm_gather_split(const AlmArgs &a, hipStream_t s)
 // components [c0, c0+npc); the values come from almtmp, or (part != nullptr)
 // from the partial sums of the ring-pair groups that reach m, added in FP64
 // in group order
-// The m of canonical row `row` (rows of m at tri_ofs(m, lmax) ..): the
-// root of m (2 lmax + 3 - m) / 2 = row, then an exact integer correction
-__device__ inline int row_m(int64_t row, int lmax)
-{
-  const double b = 2.0 * lmax + 3.0;
-  int m = (int)((b - sqrt(fmax(b * b - 8.0 * (double)row, 0.))) * 0.5);
-  while (m > 0 && tri_ofs(m, lmax) > row) --m;
-  while (tri_ofs(m + 1, lmax) <= row) ++m;
-  return m;
-}
-
 // one thread per (row, component) of the triangle: a flat grid with no idle
 // workgroups
 template <typename T>
@@ -438,8 +441,8 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
 
 void launch_alm_gather(const AlmArgs &a, hipStream_t s)
 {
-  const int64_t per = (int64_t)(a.lmax + 1) * a.ncomp;
-  const dim3 grid((unsigned)((per + 255) / 256), a.mmax + 1);
+  const int64_t tot = tri_ofs(a.mmax + 1, a.lmax) * a.ncomp;
+  const dim3 grid((unsigned)((tot + 255) / 256));
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_gather<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_gather<float>, grid, dim3(256), 0, s, a);
   check_launch("alm_gather");
