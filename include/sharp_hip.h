@@ -117,7 +117,9 @@ void sharp_hip_prepare (const sharp_geom_info *geom_info,
 /* Waits for the work enqueued on `stream` (NULL: the null stream). */
 void sharp_hip_synchronize (void *stream);
 
-/* Releases every cached table and work buffer of the current device. */
+/* Waits for the device, then releases every cached table and work buffer of
+   the current device.  Live batches keep the tables their jobs use (they
+   are freed with the batch), so a batch created before stays usable. */
 void sharp_hip_release (void);
 
 /* Per-stage timing (the reference reports only whole-call time and a flop

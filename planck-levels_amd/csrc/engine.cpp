@@ -106,7 +106,9 @@ struct GeomPlan {
     std::vector<int> hgmax;
     int *gmax = nullptr;
   };
-  std::vector<Mlim> mlims;
+  // held by pointer: JobStates keep references to entries while later jobs
+  // on the same geometry add more
+  std::vector<std::unique_ptr<Mlim>> mlims;
   // ring transforms: FFT plans grouped by LDS class, the rings too long for
   // any class (direct summation), and the list of all rings (SHARP_NO_FFT)
   RingFftPlan *fft[kFftKinds * kNumFftClasses] = {};   // [kFftKinds*class + kind]
@@ -119,7 +121,7 @@ struct GeomPlan {
   ~GeomPlan()
   {
     (void)hipFree(cth); (void)hipFree(sth); (void)hipFree(ispair); (void)hipFree(ring);
-    for (auto &m : mlims) { (void)hipFree(m.d); (void)hipFree(m.gmax); }
+    for (auto &m : mlims) { (void)hipFree(m->d); (void)hipFree(m->gmax); }
     for (auto *p : fft) (void)hipFree(p);
     (void)hipFree(btab);
     (void)hipFree(dft_list);
@@ -222,11 +224,13 @@ struct Ctx {
   Profiler prof;
   int max_lds = 65536;       // bytes of LDS one workgroup may allocate
   hipStream_t stream = nullptr;
-  std::vector<std::unique_ptr<GeomPlan>> geoms;
-  std::vector<std::unique_ptr<YlmPlan>> ylms;
-  std::vector<std::unique_ptr<NormPlan>> norms;
-  std::vector<std::unique_ptr<AlmPlan>> alms;
-  std::vector<std::unique_ptr<BsPlan>> bsplans;
+  // cached plans, shared with the jobs of live batches: sharp_hip_release
+  // drops the cache's references, a batch keeps its own until destroyed
+  std::vector<std::shared_ptr<GeomPlan>> geoms;
+  std::vector<std::shared_ptr<YlmPlan>> ylms;
+  std::vector<std::shared_ptr<NormPlan>> norms;
+  std::vector<std::shared_ptr<AlmPlan>> alms;
+  std::vector<std::shared_ptr<BsPlan>> bsplans;
   // work buffers per launch stream: calls on different streams do not share
   // them and may run concurrently
   struct StreamBufs {
@@ -351,7 +355,7 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
   std::vector<char> key = geom_key(g);
   for (auto &p : c.geoms)
     if (p->key == key) return *p;
-  auto gp = std::make_unique<GeomPlan>();
+  auto gp = std::make_shared<GeomPlan>();
   gp->key = std::move(key);
   const int P = g->npairs;
   gp->nslots = P;
@@ -466,21 +470,21 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
 static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
 {
   for (auto &m : gp.mlims)
-    if (m.lmax == lmax && m.spin == spin) return m;
-  GeomPlan::Mlim ml;
-  ml.lmax = lmax;
-  ml.spin = spin;
-  ml.h.resize(gp.nslots);
+    if (m->lmax == lmax && m->spin == spin) return *m;
+  auto ml = std::make_unique<GeomPlan::Mlim>();
+  ml->lmax = lmax;
+  ml->spin = spin;
+  ml->h.resize(gp.nslots);
   for (int k = 0; k < gp.nslots; ++k)
-    ml.h[k] = sharp_get_mlim(lmax, spin, gp.hsth[k], gp.hcth[k]);
-  ml.d = upload(ml.h);
+    ml->h[k] = sharp_get_mlim(lmax, spin, gp.hsth[k], gp.hcth[k]);
+  ml->d = upload(ml->h);
   const int gs = mix_group_slots(spin);
   const int ng = (gp.nslots + gs - 1) / gs;
-  ml.hgmax.assign(ng, -1);
-  for (int k = 0; k < gp.nslots; ++k) ml.hgmax[k / gs] = std::max(ml.hgmax[k / gs], ml.h[k]);
-  ml.gmax = upload(ml.hgmax);
+  ml->hgmax.assign(ng, -1);
+  for (int k = 0; k < gp.nslots; ++k) ml->hgmax[k / gs] = std::max(ml->hgmax[k / gs], ml->h[k]);
+  ml->gmax = upload(ml->hgmax);
   gp.mlims.push_back(std::move(ml));
-  return gp.mlims.back();
+  return *gp.mlims.back();
 }
 
 // --- recurrence tables (libsharp/sharp_ylmgen_c.c:44-198) --------------------
@@ -609,7 +613,7 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
 {
   for (auto &p : c.ylms)
     if (p->lmax == lmax && p->mmax == mmax && p->spin == spin) return *p;
-  auto yp = std::make_unique<YlmPlan>();
+  auto yp = std::make_shared<YlmPlan>();
   yp->lmax = lmax;
   yp->mmax = mmax;
   yp->spin = spin;
@@ -678,7 +682,7 @@ static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
   for (auto &p : c.norms)
     if (p->lmax == lmax && p->spin == spin && p->deriv == deriv) return p->d;
   const std::vector<double> n = norm_values(lmax, spin, deriv);
-  auto np = std::make_unique<NormPlan>();
+  auto np = std::make_shared<NormPlan>();
   np->lmax = lmax;
   np->spin = spin;
   np->deriv = deriv;
@@ -691,7 +695,7 @@ static BsPlan &get_bsplan(Ctx &c, int lmax, int mmax, int spin, int ncol)
 {
   for (auto &p : c.bsplans)
     if (p->lmax == lmax && p->mmax == mmax && p->spin == spin && p->ncol == ncol) return *p;
-  auto bp = std::make_unique<BsPlan>();
+  auto bp = std::make_shared<BsPlan>();
   bp->lmax = lmax;
   bp->mmax = mmax;
   bp->spin = spin;
@@ -722,7 +726,7 @@ static int64_t *get_almplan(Ctx &c, const sharp_alm_info *a, int mmax)
     if (p->key == key) return p->mvstart_by_m;
   std::vector<int64_t> mvs(mmax + 1);
   for (int mi = 0; mi < a->nm; ++mi) mvs[a->mval[mi]] = a->mvstart[mi];
-  auto ap = std::make_unique<AlmPlan>();
+  auto ap = std::make_shared<AlmPlan>();
   ap->key = std::move(key);
   ap->mvstart_by_m = upload(mvs);
   c.alms.push_back(std::move(ap));
@@ -830,7 +834,19 @@ struct JobState {
   int ncomp_alm = 0, ncomp_map = 0;
   double triples = 0., c0 = 0., c1 = 0.;
   double ring_bytes_fft = 0., ring_bytes_all = 0., ring_bytes_dft = 0., alm_bytes = 0.;
+  // the cached plans the arguments above point into, held for as long as
+  // the state lives (a batch outlives sharp_hip_release)
+  std::vector<std::shared_ptr<void>> keep;
 };
+
+// the cache entry for which is(entry) holds
+template <typename T, typename Pred>
+static std::shared_ptr<void> owner(const std::vector<std::shared_ptr<T>> &v, Pred is)
+{
+  for (auto &e : v)
+    if (is(*e)) return e;
+  SHP_FAIL("internal error: plan not in the cache");
+}
 
 static size_t phase_elems(const Job &j, const GeomPlan &gp)
 {
@@ -889,6 +905,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   int64_t *d_mvs = get_almplan(c, j.a, j.mmax);
   s.gp = &gp;
   s.ml = &ml;
+  s.keep = {owner(c.geoms, [&](const GeomPlan &x) { return &x == &gp; }),
+            owner(c.ylms, [&](const YlmPlan &x) { return &x == &yp; }),
+            owner(c.norms, [&](const NormPlan &x) { return x.d == d_norm; }),
+            owner(c.alms, [&](const AlmPlan &x) { return x.mvstart_by_m == d_mvs; })};
   const unsigned long long ops = op_count(j, gp, ml);
   if (opcnt) *opcnt = ops;
 

@@ -455,6 +455,12 @@ def device_count():
     return LIB.sharp_hip_device_count()
 
 
+def release():
+    """Frees the cached tables and work buffers of the current device (live
+    batches keep what their jobs use)."""
+    LIB.sharp_hip_release()
+
+
 def synchronize(stream=None):
     LIB.sharp_hip_synchronize(None if stream is None else ct.c_void_p(stream.cuda_stream))
 
