@@ -110,9 +110,33 @@ void sharp_hip_batch_config (const sharp_hip_batch *batch, int *tgroup,
 void sharp_hip_batch_destroy (sharp_hip_batch *batch);
 
 /* Builds (and caches) the device tables for this geometry / a_lm layout /
-   spin outside any timed region.  Optional: the first transform does it. */
+   spin outside any timed region.  Optional: the first transform does it.
+   Also finds the current device's scratch area of
+   sharp_hip_execute_device_noalloc. */
 void sharp_hip_prepare (const sharp_geom_info *geom_info,
   const sharp_alm_info *alm_info, int spin);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): sharp_hip_execute_device with the same arguments and
+   results, which calls no hipMalloc, hipFree or other allocating routine.
+   Its work memory (phase array, canonical a_lm, analysis partial sums, BF16
+   pieces) comes from a fixed scratch area of SHARP_HIP_NOALLOC_SCRATCH_BYTES
+   per GPU, a device array each GPU holds from the first sharp_hip_prepare on
+   it until the process ends.  Requirements, each checked with a clear error
+   (message, then exit(1)):
+   * sharp_hip_prepare(geom_info, alm_info, spin) was called on this device
+     (spin 1 for SHARP_ALM2MAP_DERIV1) since the last sharp_hip_release;
+   * the job's work memory fits the scratch area (about Nside 256, lmax 512,
+     TQU, one transform; the message states the bytes needed).
+   All calls on one GPU share its scratch area, which keeps the last call's
+   contents: calls that overlap in time on one GPU (different streams,
+   threads) race, so serialise them.  Per-stage profiling skips these calls. */
+#define SHARP_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+void sharp_hip_execute_device_noalloc (sharp_jobtype type, int spin,
+  void *alm, void *map, const sharp_geom_info *geom_info,
+  const sharp_alm_info *alm_info, int ntrans, int flags, void *stream,
+  unsigned long long *opcnt);
 
 /* Waits for the work enqueued on `stream` (NULL: the null stream). */
 void sharp_hip_synchronize (void *stream);

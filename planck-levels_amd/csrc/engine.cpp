@@ -253,6 +253,20 @@ struct Ctx {
 
 static std::mutex g_mutex;
 static std::map<int, std::unique_ptr<Ctx>> g_ctx;
+// Set (under g_mutex) while sharp_hip_execute_device_noalloc runs: a table
+// that is not cached yet is then an error instead of being built.
+static bool g_noalloc = false;
+
+static void need_build(const char *what)
+{
+  if (!g_noalloc) return;
+  char b[256];
+  std::snprintf(b, sizeof(b),
+                "sharp_hip_execute_device_noalloc: the %s of this job is not prepared "
+                "(call sharp_hip_prepare with this geometry, a_lm layout and spin first)",
+                what);
+  SHP_FAIL(b);
+}
 
 static Ctx &context()
 {
@@ -355,6 +369,7 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
   std::vector<char> key = geom_key(g);
   for (auto &p : c.geoms)
     if (p->key == key) return *p;
+  need_build("geometry");
   auto gp = std::make_shared<GeomPlan>();
   gp->key = std::move(key);
   const int P = g->npairs;
@@ -471,6 +486,7 @@ static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
 {
   for (auto &m : gp.mlims)
     if (m->lmax == lmax && m->spin == spin) return *m;
+  need_build("per-ring m limit table");
   auto ml = std::make_unique<GeomPlan::Mlim>();
   ml->lmax = lmax;
   ml->spin = spin;
@@ -579,6 +595,7 @@ static void unit_table0(const YlmPlan &yp, std::vector<double> &tab, std::vector
 static void ensure_unit_table(YlmPlan &yp)
 {
   if (yp.tab1) return;
+  need_build("mixed-precision recurrence table");
   const int lmax = yp.lmax, mmax = yp.mmax, s = yp.spin;
   std::vector<double> tab, lsc((size_t)tri_ofs(mmax + 1, lmax), 1.);
   if (s == 0) {
@@ -613,6 +630,7 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
 {
   for (auto &p : c.ylms)
     if (p->lmax == lmax && p->mmax == mmax && p->spin == spin) return *p;
+  need_build("recurrence table");
   auto yp = std::make_shared<YlmPlan>();
   yp->lmax = lmax;
   yp->mmax = mmax;
@@ -681,6 +699,7 @@ static double *get_norm(Ctx &c, int lmax, int spin, int deriv)
 {
   for (auto &p : c.norms)
     if (p->lmax == lmax && p->spin == spin && p->deriv == deriv) return p->d;
+  need_build("norm_l table");
   const std::vector<double> n = norm_values(lmax, spin, deriv);
   auto np = std::make_shared<NormPlan>();
   np->lmax = lmax;
@@ -695,6 +714,7 @@ static BsPlan &get_bsplan(Ctx &c, int lmax, int mmax, int spin, int ncol)
 {
   for (auto &p : c.bsplans)
     if (p->lmax == lmax && p->mmax == mmax && p->spin == spin && p->ncol == ncol) return *p;
+  need_build("BF16 piece table");
   auto bp = std::make_shared<BsPlan>();
   bp->lmax = lmax;
   bp->mmax = mmax;
@@ -724,6 +744,7 @@ static int64_t *get_almplan(Ctx &c, const sharp_alm_info *a, int mmax)
   put(a->mvstart, sizeof(ptrdiff_t) * a->nm);
   for (auto &p : c.alms)
     if (p->key == key) return p->mvstart_by_m;
+  need_build("a_lm layout");
   std::vector<int64_t> mvs(mmax + 1);
   for (int mi = 0; mi < a->nm; ++mi) mvs[a->mval[mi]] = a->mvstart[mi];
   auto ap = std::make_shared<AlmPlan>();
@@ -1104,10 +1125,28 @@ static bool fp64_anal_reduce()
   return e && std::strcmp(e, "reduce") == 0;
 }
 
+// transforms per analysis launch of the matrix-core kernels
+static int anal_tmax(const Job &j)
+{
+  return is_mixed(j.flags) ? ((j.spin == 0) ? kMixMaxT0 : kMixMaxTS)
+                           : ((j.spin == 0) ? kF64MaxT0 : kF64MaxTS);
+}
+
+// bytes of the per-group partial sums of one analysis launch (ng groups of
+// ring pairs), 0 when the job needs none
+static size_t part_bytes(const Job &j, int ng)
+{
+  if (j.type == SHARP_ALM2MAP || (!is_mixed(j.flags) && fp64_anal_reduce())) return 0;
+  const size_t esz = is_mixed(j.flags) ? sizeof(float) : sizeof(double);
+  const int npc = std::min(anal_tmax(j), j.ntrans) * j.nalm;
+  return (size_t)ng * (size_t)tri_ofs(j.mmax + 1, j.lmax) * 2 * npc * esz;
+}
+
 // Analysis Legendre stage and a_lm scatter of one job, in transform groups:
-// the matrix-core kernels write per-group partial sums into `part`
-// (FP32 for the mixed form, FP64 for the FP64 form) and the scatter adds them.
-static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
+// the matrix-core kernels write per-group partial sums into `part` (of
+// part_bytes(j, groups) bytes; FP32 for the mixed form, FP64 for the FP64
+// form) and the scatter adds them.
+static void stage_analysis(Ctx &c, JobState &s, void *part, hipStream_t st)
 {
   const Job &j = s.j;
   const bool mixed = is_mixed(j.flags);
@@ -1116,11 +1155,7 @@ static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
     timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
     return;
   }
-  const int tmax = mixed ? ((j.spin == 0) ? kMixMaxT0 : kMixMaxTS)
-                         : ((j.spin == 0) ? kF64MaxT0 : kF64MaxTS);
-  const size_t esz = mixed ? sizeof(float) : sizeof(double);
-  const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
-  const int ng = (int)s.ml->hgmax.size();
+  const int tmax = anal_tmax(j);
   const char *name = mixed ? mixed_name(j)
                            : (j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin");
   LegArgs &la = s.la;
@@ -1128,7 +1163,7 @@ static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
     la.t0 = t;
     la.ntl = std::min(tmax, j.ntrans - t);
     const int npc = la.ntl * j.nalm;
-    la.part = part.get((size_t)ng * nrows * 2 * npc * esz);
+    la.part = part;
     la.part_ld = 2 * npc;
     la.part_col0 = 0;
     timed(c, st, name, s.triples * (s.c0 + s.c1 * la.ntl), 0.,
@@ -1144,33 +1179,68 @@ static void stage_analysis(Ctx &c, JobState &s, DevBuf &part, hipStream_t st)
   }
 }
 
+// Work memory of one call supplied by the caller of run_device instead of
+// the per-stream buffers (sharp_hip_execute_device_noalloc).
+struct WorkPtrs {
+  double2 *phase = nullptr, *almtmp = nullptr;
+  void *part = nullptr;          // part_bytes
+  unsigned *bsplit = nullptr;    // bs_words: the BF16 pieces of the a_lm
+  int64_t *bs_base = nullptr;    // [mmax+1] word offsets (filled on the device)
+};
+
+// BF16 spin-s synthesis: its a_lm are split into BF16 pieces by the gather
+static bool uses_bsplit(const Job &j)
+{
+  return is_mixed(j.flags) && mixed_contr(j.flags) == kContrBf16x3 && j.spin > 0 &&
+         j.type == SHARP_ALM2MAP;
+}
+static int bs_ncol(const Job &j) { return 2 * j.ntrans * j.nalm + 1; }
+// 32-bit words of the BF16 pieces (the layout of bs_pairs)
+static int64_t bs_words(const Job &j)
+{
+  int64_t t = 0;
+  for (int m = 0; m <= j.mmax; ++m)
+    t += (int64_t)3 * bs_ncol(j) * bs_pairs(j.lmax, std::max(m, j.spin));
+  return t;
+}
+
 static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
                        int64_t alm_base, int64_t map_base, hipStream_t st,
-                       unsigned long long *opcnt)
+                       unsigned long long *opcnt, const WorkPtrs *w = nullptr)
 {
   GeomPlan &gp = get_geom(c, j.g);
-  Ctx::StreamBufs &sb = c.bufs(st);
-  double2 *phase = static_cast<double2 *>(sb.phase.get(phase_elems(j, gp) * sizeof(double2)));
-  double2 *almtmp = static_cast<double2 *>(sb.almtmp.get(almtmp_elems(j) * sizeof(double2)));
+  Ctx::StreamBufs *sb = w ? nullptr : &c.bufs(st);
+  double2 *phase = w ? w->phase
+                     : static_cast<double2 *>(sb->phase.get(phase_elems(j, gp) * sizeof(double2)));
+  double2 *almtmp = w ? w->almtmp
+                      : static_cast<double2 *>(sb->almtmp.get(almtmp_elems(j) * sizeof(double2)));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
 
   const bool mixed = is_mixed(j.flags);
   if (j.type != SHARP_ALM2MAP) {
     stage_rings_to_phase(c, s, st);
-    stage_analysis(c, s, sb.part, st);
+    void *part = w ? w->part : sb->part.get(part_bytes(j, (int)s.ml->hgmax.size()));
+    stage_analysis(c, s, part, st);
     return;
   }
-  if (mixed && mixed_contr(j.flags) == kContrBf16x3 && j.spin > 0) {
-    // the BF16 spin-s synthesis reads the a_lm split into BF16 pieces by the
-    // gather
-    const int ncol = 2 * s.la.ncomp_alm + 1;
-    BsPlan &bp = get_bsplan(c, j.lmax, j.mmax, j.spin, ncol);
-    unsigned *bs = static_cast<unsigned *>(sb.bsplit.get((size_t)bp.total * sizeof(unsigned)));
+  if (uses_bsplit(j)) {
+    const int ncol = bs_ncol(j);
+    unsigned *bs;
+    const int64_t *base;
+    if (w) {
+      bs = w->bsplit;
+      launch_bs_base(w->bs_base, j.lmax, j.mmax, j.spin, ncol, st);
+      base = w->bs_base;
+    } else {
+      BsPlan &bp = get_bsplan(c, j.lmax, j.mmax, j.spin, ncol);
+      bs = static_cast<unsigned *>(sb->bsplit.get((size_t)bp.total * sizeof(unsigned)));
+      base = bp.d;
+    }
     s.aa.bsplit = bs;
-    s.aa.bs_base = bp.d;
+    s.aa.bs_base = base;
     s.aa.bs_ncol = ncol;
     s.la.bsplit = bs;
-    s.la.bs_base = bp.d;
+    s.la.bs_base = base;
     s.la.bs_ncol = ncol;
   }
   if (s.aa.bsplit)
@@ -1379,7 +1449,7 @@ static void batch_run(Ctx &c, BatchPlan &b, hipStream_t st)
         stage_phase_to_rings(c, s, st);
       } else {
         stage_rings_to_phase(c, s, st);
-        stage_analysis(c, s, b.part[i], st);
+        stage_analysis(c, s, b.part[i].get(part_bytes(s.j, (int)s.ml->hgmax.size())), st);
       }
     }
     return;
@@ -1622,6 +1692,59 @@ void sharp_hip_execute_device(sharp_jobtype type, int spin, void *alm, void *map
              0, 0, st, opcnt);
 }
 
+void sharp_hip_execute_device_noalloc(sharp_jobtype type, int spin, void *alm, void *map,
+                                      const sharp_geom_info *geom_info,
+                                      const sharp_alm_info *alm_info, int ntrans, int flags,
+                                      void *stream, unsigned long long *opcnt)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  const Job j = make_job(type, spin, geom_info, alm_info, ntrans, flags);
+  // the call's work memory, laid out in the scratch area; sized from the job
+  // alone and checked before any device call
+  const int nslots = geom_info->npairs;
+  const int gs = mix_group_slots(j.spin);
+  const int nmb = (j.mmax + MB) / MB;
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t off = 0;
+  const size_t o_phase = off;
+  off += align((size_t)j.ntrans * j.nmaps * nmb * (size_t)nslots * 2 * MB * sizeof(double2));
+  const size_t o_almtmp = off;
+  off += align(almtmp_elems(j) * sizeof(double2));
+  const size_t o_part = off;
+  off += align(part_bytes(j, (nslots + gs - 1) / gs));
+  const size_t o_bsplit = off, o_bsbase = off + (uses_bsplit(j) ? align((size_t)bs_words(j) * 4) : 0);
+  if (uses_bsplit(j)) off = o_bsbase + align((size_t)(j.mmax + 1) * sizeof(int64_t));
+  if (off > kNoallocScratchBytes) {
+    char b[320];
+    std::snprintf(b, sizeof(b),
+                  "sharp_hip_execute_device_noalloc: this job needs %zu bytes of work memory, "
+                  "more than the %zu of the fixed scratch area (use sharp_hip_execute_device)",
+                  off, kNoallocScratchBytes);
+    SHP_FAIL(b);
+  }
+  int dev = 0;
+  char *base = nullptr;
+  if (hipGetDevice(&dev) == hipSuccess && g_ctx.count(dev))
+    base = static_cast<char *>(noalloc_scratch(dev));
+  if (!base)
+    SHP_FAIL("sharp_hip_execute_device_noalloc: call sharp_hip_prepare on this device first");
+  Ctx &c = context();
+  WorkPtrs w;
+  w.phase = reinterpret_cast<double2 *>(base + o_phase);
+  w.almtmp = reinterpret_cast<double2 *>(base + o_almtmp);
+  w.part = base + o_part;
+  w.bsplit = reinterpret_cast<unsigned *>(base + o_bsplit);
+  w.bs_base = reinterpret_cast<int64_t *>(base + o_bsbase);
+  // no per-stage events either: the profiler creates them on demand
+  const bool prof = c.prof.on;
+  c.prof.on = false;
+  g_noalloc = true;
+  run_device(c, j, static_cast<void *const *>(alm), static_cast<void *const *>(map), 0, 0,
+             static_cast<hipStream_t>(stream), opcnt, &w);
+  g_noalloc = false;
+  c.prof.on = prof;
+}
+
 void sharp_hip_alm2map_adjoint_device(int spin, void *alm, void *map,
                                       const sharp_geom_info *geom_info,
                                       const sharp_alm_info *alm_info, int ntrans,
@@ -1820,7 +1943,10 @@ void sharp_hip_prepare(const sharp_geom_info *geom_info,
   // libsharp's table (FP64 form) and the unit-coefficient one (mixed forms)
   ensure_unit_table(get_ylm(c, alm_info->lmax, mmax, spin));
   get_norm(c, alm_info->lmax, spin, 0);
+  if (spin == 1) get_norm(c, alm_info->lmax, spin, 1);   // ALM2MAP_DERIV1
   get_almplan(c, alm_info, mmax);
+  // the address of this device's scratch area (sharp_hip_execute_device_noalloc)
+  noalloc_setup();
 }
 
 void sharp_hip_synchronize(void *stream)

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include "shp_internal.hpp"
+#include "sharp_hip.h"
 
 namespace shp {
 
@@ -189,5 +190,16 @@ void launch_alm_scatter(const AlmArgs &a, hipStream_t st);
 // the gather of a BF16 spin-s synthesis: FP32 rows and their BF16 pieces
 // (bs_pairs)
 void launch_alm_gather_split(const AlmArgs &a, hipStream_t st);
+
+// noalloc.hip: the fixed scratch area of sharp_hip_execute_device_noalloc,
+// one copy per GPU.  noalloc_setup() finds the current device's copy (once
+// per device); noalloc_scratch(dev) returns it, or nullptr before set-up.
+// Both are called with the engine's mutex held.
+constexpr size_t kNoallocScratchBytes = SHARP_HIP_NOALLOC_SCRATCH_BYTES;
+void noalloc_setup();
+void *noalloc_scratch(int device);
+// base[m] = word offset of m's BF16 pieces (the layout of bs_pairs), for
+// m = 0..mmax
+void launch_bs_base(int64_t *base, int lmax, int mmax, int spin, int ncol, hipStream_t st);
 
 } // namespace shp

@@ -97,6 +97,9 @@ def _load():
     lib.sharp_hip_execute_device.argtypes = [
         ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int,
         ct.c_void_p, ct.POINTER(ct.c_ulonglong)]
+    lib.sharp_hip_execute_device_noalloc.argtypes = [
+        ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, G, A, ct.c_int, ct.c_int,
+        ct.c_void_p, ct.POINTER(ct.c_ulonglong)]
     for f in ("sharp_hip_alm2map_adjoint_device",
               "sharp_hip_map2alm_adjoint_device"):
         getattr(lib, f).argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, G, A,
@@ -261,6 +264,29 @@ def execute_device(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags,
     LIB.sharp_hip_execute_device(jobtype, spin, ap, mp, geom.ptr, ainfo.ptr,
                                  ntrans, flags, ct.c_void_p(stream.cuda_stream),
                                  ct.byref(op))
+    return op.value
+
+
+def prepare(geom, ainfo, spin):
+    """Builds the device tables of (geometry, a_lm layout, spin) ahead of the
+    first transform (sharp_hip_prepare)."""
+    LIB.sharp_hip_prepare(geom.ptr, ainfo.ptr, spin)
+
+
+def execute_device_noalloc(jobtype, spin, alms, maps, geom, ainfo, ntrans, flags,
+                           stream=None):
+    """execute_device through sharp_hip_execute_device_noalloc: no device
+    allocation; needs prepare(geom, ainfo, spin) first and a job whose work
+    memory fits the library's fixed scratch area."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    ap = (ct.c_void_p * len(alms))(*[a.data_ptr() for a in alms])
+    mp = (ct.c_void_p * len(maps))(*[m.data_ptr() for m in maps])
+    op = ct.c_ulonglong(0)
+    LIB.sharp_hip_execute_device_noalloc(jobtype, spin, ap, mp, geom.ptr, ainfo.ptr,
+                                         ntrans, flags, ct.c_void_p(stream.cuda_stream),
+                                         ct.byref(op))
     return op.value
 
 

@@ -177,6 +177,22 @@ def test_error_behaviour(code, msg):
     assert msg in r.stderr, r.stderr
 
 
+def test_noalloc_rejects_oversize_job():
+    """sharp_hip_execute_device_noalloc refuses a job whose work memory does
+    not fit its fixed scratch area, with the sizes, before any device call."""
+    r = _run_py("""
+        import ctypes as ct
+        g=L.healpix(1024); a=L.triangular(2048)
+        f=L.lib.sharp_hip_execute_device_noalloc
+        f.argtypes=[ct.c_int,ct.c_int,ct.c_void_p,ct.c_void_p,ct.POINTER(O.GeomInfo),
+                    ct.POINTER(O.AlmInfo),ct.c_int,ct.c_int,ct.c_void_p,ct.c_void_p]
+        p=(ct.c_void_p*1)()
+        f(1,0,p,p,g,a,1,O.DP,None,None)
+        """)
+    assert r.returncode == 1, (r.returncode, r.stderr)
+    assert "more than the %d of the fixed scratch area" % (64 << 20) in r.stderr, r.stderr
+
+
 def test_fails_loudly_without_gpu():
     """No CPU fallback: a valid transform on a host without a GPU stops with a
     message instead of computing anything."""
