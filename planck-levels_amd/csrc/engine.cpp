@@ -5,7 +5,8 @@
 // Job normalisation and checks follow libsharp/sharp.c:238-251,762-880 and
 // libsharp/sharp_ylmgen_c.c:44-52; the recurrence tables are built exactly as
 // libsharp/sharp_ylmgen_c.c:44-232 builds them (same operations in the same
-// order), so both implementations start from bit-identical coefficients; the
+// order), so both implementations start from the same coefficients (up to
+// the regrouping of products -ffast-math allows the reference build); the
 // spin-s f1 is stored premultiplied by f0 (one product more, see rec_step).
 //
 // Pipeline per call (one HIP stream, no host synchronisation inside):
@@ -33,6 +34,7 @@
 #include "sharp_hip.h"
 #include "shp_internal.hpp"
 #include "kernels.hpp"
+#include "ylm_tables.hpp"
 
 namespace shp {
 
@@ -47,7 +49,6 @@ namespace shp {
     }                                                                          \
   } while (0)
 
-static const double kPi = 3.141592653589793238462643383279502884197;
 
 // ---------------------------------------------------------------------------
 // device memory helpers
@@ -504,62 +505,32 @@ static GeomPlan::Mlim &get_mlim(GeomPlan &gp, int lmax, int spin)
 }
 
 // --- recurrence tables (libsharp/sharp_ylmgen_c.c:44-198) --------------------
-static void host_normalize(double &val, int &scale, double xfmax)
-{
-  while (std::fabs(val) > xfmax) { val *= kFSmall; ++scale; }
-  if (val != 0.)
-    while (std::fabs(val) < xfmax * kFSmall) { val *= kFBig; --scale; }
-}
-
 // the spin-s recurrence coefficients (f0, f0 f1, f2) of entry l at
-// 3 (tab_ofs(m, lmax) + l - m), libsharp/sharp_ylmgen_c.c:150-175
+// 3 (tab_ofs(m, lmax) + l - m)
 static std::vector<double> spin_table(int lmax, int mmax, int s)
 {
   std::vector<double> tab(3 * (size_t)tab_ofs(mmax + 1, lmax), 0.);
-  std::vector<double> flm1(2 * lmax + 1), flm2(2 * lmax + 1), inv(lmax + 1);
-  inv[0] = 0;
-  for (int k = 1; k < lmax + 1; ++k) inv[k] = 1. / k;
-  for (int k = 0; k < 2 * lmax + 1; ++k) {
-    flm1[k] = std::sqrt(1. / (k + 1.));
-    flm2[k] = std::sqrt(k / (k + 1.));
-  }
+  const SpinFactors f(lmax);
   for (int m = 0; m <= mmax; ++m) {
     double *row = tab.data() + 3 * tab_ofs(m, lmax);
-    const int mhi = std::max(m, s);
-    for (int l = mhi; l < lmax; ++l) {
-      double t = flm1[l + m] * flm1[l - m] * flm1[l + s] * flm1[l - s];
-      const double lt = 2 * l + 1, l1 = l + 1;
+    ylm_spin_coeffs(m, s, lmax, f.flm1.data(), f.flm2.data(), f.inv.data(), row, m);
+    // stored as f0*f1: the kernels form (cth -+ f1) f0 as one FMA
+    for (int l = std::max(m, s); l < lmax; ++l) {
       double *e = row + 3 * (l + 1 - m);
-      e[0] = l1 * lt * t;
-      // stored as f0*f1: the kernels form (cth -+ f1) f0 as one FMA
-      e[1] = e[0] * (m * s * inv[l] * inv[l + 1]);
-      t = flm2[l + m] * flm2[l - m] * flm2[l + s] * flm2[l - s];
-      e[2] = t * l1 * inv[l];
+      e[1] = e[0] * e[1];
     }
   }
   return tab;
 }
 
 // the spin-0 recurrence coefficients (a_l, b_l) of entry l at
-// 2 (tab_ofs(m, lmax) + l - m), libsharp/sharp_ylmgen_c.c:88-110
+// 2 (tab_ofs(m, lmax) + l - m)
 static std::vector<double> spin0_table(int lmax, int mmax)
 {
-  std::vector<double> tab(2 * (size_t)tab_ofs(mmax + 1, lmax), 0.), root(2 * lmax + 5),
-      iroot(2 * lmax + 5);
-  for (int k = 0; k < 2 * lmax + 5; ++k) {
-    root[k] = std::sqrt((double)k);
-    iroot[k] = (k == 0) ? 0. : 1. / root[k];
-  }
-  for (int m = 0; m <= mmax; ++m) {
-    double *row = tab.data() + 2 * tab_ofs(m, lmax);
-    row[0] = root[2 * m + 3];
-    row[1] = 0.;
-    for (int l = m + 1; l <= lmax; ++l) {
-      const double tmp = root[2 * l + 3] * iroot[l + 1 + m] * iroot[l + 1 - m];
-      row[2 * (l - m)] = tmp * root[2 * l + 1];
-      row[2 * (l - m) + 1] = tmp * root[l + m] * root[l - m] * iroot[2 * l - 1];
-    }
-  }
+  std::vector<double> tab(2 * (size_t)tab_ofs(mmax + 1, lmax), 0.), root, iroot;
+  ylm_roots(lmax, root, iroot);
+  for (int m = 0; m <= mmax; ++m)
+    ylm_spin0_coeffs(m, lmax, root.data(), iroot.data(), tab.data() + 2 * tab_ofs(m, lmax));
   return tab;
 }
 
@@ -638,37 +609,14 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
   if (spin == 0) {
     std::vector<double> tab = spin0_table(lmax, mmax);
     std::vector<double> mfac(mmax + 1);
-    mfac[0] = 0.2820947917738781434740397257803862929220;   // 1/sqrt(4 pi)
-    for (int m = 1; m <= mmax; ++m)
-      mfac[m] = mfac[m - 1] * std::sqrt((2 * m + 1.) / (2 * m));
+    ylm_mfac(mmax, mfac.data());
     yp->tab = upload(tab);
     yp->mfac = upload(mfac);
   } else {
-    const int s = spin;
-    std::vector<double> tab = spin_table(lmax, mmax, s);
-    std::vector<double> fac(2 * lmax + 1);
-    std::vector<int> facscale(2 * lmax + 1);
-    fac[0] = 1;
-    facscale[0] = 0;
-    for (int k = 1; k < 2 * lmax + 1; ++k) {
-      fac[k] = fac[k - 1] * std::sqrt((double)k);
-      facscale[k] = facscale[k - 1];
-      host_normalize(fac[k], facscale[k], kFBigHalf);
-    }
+    std::vector<double> tab = spin_table(lmax, mmax, spin);
     std::vector<double> prefac(mmax + 1);
     std::vector<int> fscale(mmax + 1);
-    for (int m = 0; m <= mmax; ++m) {
-      int mlo = s, mhi = m;
-      if (mhi < mlo) std::swap(mhi, mlo);
-      double tfac = fac[2 * mhi] / fac[mhi + mlo];
-      int tscale = facscale[2 * mhi] - facscale[mhi + mlo];
-      host_normalize(tfac, tscale, kFBigHalf);
-      tfac /= fac[mhi - mlo];
-      tscale -= facscale[mhi - mlo];
-      host_normalize(tfac, tscale, kFBigHalf);
-      prefac[m] = tfac;
-      fscale[m] = tscale;
-    }
+    ylm_spin_prefactors(lmax, mmax, spin, prefac.data(), fscale.data());
     yp->tab = upload(tab);
     yp->prefac = upload(prefac);
     yp->fscale = upload(fscale);
@@ -681,17 +629,7 @@ static YlmPlan &get_ylm(Ctx &c, int lmax, int mmax, int spin)
 static std::vector<double> norm_values(int lmax, int spin, int deriv)
 {
   std::vector<double> n(lmax + 1);
-  if (deriv) {
-    for (int l = 0; l <= lmax; ++l)
-      n[l] = (l < 1) ? 0. : 0.5 * std::sqrt(l * (l + 1.) * (2 * l + 1.) / (4 * kPi));
-  } else if (spin == 0) {
-    for (int l = 0; l <= lmax; ++l) n[l] = 1.;
-  } else {
-    double spinsign = (spin > 0) ? -1.0 : 1.0;
-    spinsign = (spin & 1) ? -spinsign : spinsign;
-    for (int l = 0; l <= lmax; ++l)
-      n[l] = (l < spin) ? 0. : spinsign * 0.5 * std::sqrt((2 * l + 1) / (4 * kPi));
-  }
+  ylm_norm(lmax, spin, deriv != 0, n.data());
   return n;
 }
 
