@@ -29,6 +29,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "planck-levels_amd"))
+from sht_inputs import make_alms, spin0_alms  # noqa: E402
 
 METRIC = ("alm2map+map2alm/s, Nside 2048 lmax 4096 TQU; mixed vs FP64 at 1e-6 "
           "rel err")
@@ -37,52 +38,6 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def spectra(lmax, fwhm_arcmin=5.0):
-    l = np.arange(lmax + 1, dtype=np.float64)
-    sigma = fwhm_arcmin / 60.0 * math.pi / 180.0 / math.sqrt(8.0 * math.log(2.0))
-    bl2 = np.exp(-l * (l + 1) * sigma * sigma)
-    tt = bl2 / (l * (l + 1) + 1)
-    ee = 0.1 * tt
-    bb = 0.01 * tt
-    te = 0.5 * np.sqrt(tt * ee)
-    return tt, ee, bb, te
-
-
-def make_alms(torch, lmax, batch, seed, dev):
-    """Triangular (Healpix) layout, complex128, on the GPU."""
-    tt, ee, bb, te = spectra(lmax)
-    m_of = np.concatenate([np.full(lmax + 1 - m, m) for m in range(lmax + 1)])
-    l_of = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
-    l_t = torch.from_numpy(l_of).to(dev)
-    m0 = torch.from_numpy(m_of == 0).to(dev)
-    ctt = torch.from_numpy(tt).to(dev)[l_t]
-    cee = torch.from_numpy(ee).to(dev)[l_t]
-    cbb = torch.from_numpy(bb).to(dev)[l_t]
-    cte = torch.from_numpy(te).to(dev)[l_t]
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    n = len(l_of)
-
-    def cgauss():
-        re = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
-        im = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
-        # m>0: Re, Im ~ N(0, 1/2); m=0: real N(0, 1)
-        s = torch.where(m0, torch.ones_like(re), torch.full_like(re, math.sqrt(0.5)))
-        im = torch.where(m0, torch.zeros_like(im), im)
-        return torch.complex(re * s, im * s)
-
-    T, E, B = [], [], []
-    for _ in range(batch):
-        g1, g2, g3 = cgauss(), cgauss(), cgauss()
-        at = torch.sqrt(ctt) * g1
-        ae = (cte / torch.sqrt(ctt)) * g1 + torch.sqrt(torch.clamp(cee - cte * cte / ctt, min=0)) * g2
-        ab = torch.sqrt(cbb) * g3
-        T.append(at.contiguous())
-        E.append(ae.contiguous())
-        B.append(ab.contiguous())
-    return T, E, B
 
 
 def cpu_baseline(nside, lmax, realisation, nreal, min_seconds=10.0):
@@ -353,14 +308,7 @@ def batch_workload(S, torch, dev, args):
     for kind, n, lmax, nt in specs:
         geom = S.Geometry.healpix(n) if kind == "healpix" else S.Geometry.iso("gauss", lmax + 1, 2 * lmax + 1)
         ainfo = S.AlmInfo.triangular(lmax)
-        l_of = torch.cat([torch.arange(m, lmax + 1, device=dev) for m in range(lmax + 1)]).double()
-        sd = torch.sqrt(1.0 / (l_of * (l_of + 1) + 1) / 2)
-        alms = []
-        for _ in range(nt):
-            re = torch.randn(l_of.shape, generator=gen, device=dev, dtype=torch.float64)
-            im = torch.randn(l_of.shape, generator=gen, device=dev, dtype=torch.float64)
-            im[: lmax + 1] = 0.0
-            alms.append(torch.complex(re * sd, im * sd))
+        alms = spin0_alms(torch, lmax, nt, gen, dev)
         npix = geom.npix
         jobs.append({"name": "%s %d lmax %d x%d" % (kind, n, lmax, nt), "geom": geom,
                      "ainfo": ainfo, "nt": nt, "alms": alms,
