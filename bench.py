@@ -40,53 +40,100 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nside, lmax, realisation, nreal, min_seconds=10.0):
-    """Reference libsharp (OpenMP) on TQU realisations 0, 1, ... of the batch,
-    alm2map + map2alm each, until min_seconds have accumulated (at most
-    nreal).  Returns (dict, maps, alms) with the outputs of realisation 0,
-    or (None, None, None) without the build.  realisation(i) gives the
-    (T, E, B) a_lm of realisation i as numpy arrays."""
+def cpu_info():
+    """(model name, physical cores) of this host from /proc/cpuinfo."""
+    model, cores = None, set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return model, (len(cores) or os.cpu_count())
+
+
+def cpu_baseline(nside, lmax, realisation, nreal, min_seconds=2.0, batched=True):
+    """Reference libsharp (oracle/_ref, OpenMP) on the TQU workload, timed as
+    its own benchmark does (libsharp/sharp_testsuite.c:554-577): one warm-up
+    per job key (fills libsharp's nv timing oracle), then tries until at
+    least two have run and min_seconds have accumulated; the best try is
+    reported, with GFLOP/s from the opcnt libsharp returns.  One try = one
+    realisation's alm2map (spin 0 + spin 2) + map2alm, ntrans = 1.  With
+    `batched`, also one round trip of all nreal realisations as ntrans =
+    nreal calls (libsharp shares the recurrence over them,
+    libsharp/sharp_core.c:76-240).  Returns (dict, maps, alms) with the
+    outputs of realisation 0, or (None, None, None) without the build.
+    realisation(i) gives the (T, E, B) a_lm of realisation i (numpy)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.load_ref()
     if ref is None:
         return None, None, None
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    model, phys = cpu_info()
     g = ref.healpix(nside)
     a = ref.triangular(lmax)
     npix = 12 * nside * nside
-    # warm-up (fills libsharp's nv timing oracle) on a small job
+    # warm-up on a small job: the first call per (type, spin, ntrans) runs
+    # libsharp's nv oracle
     gs = ref.healpix(16)
     as_ = ref.triangular(32)
-    for sp, na in ((0, 1), (2, 2)):
-        ref.execute(O.ALM2MAP, sp, [np.zeros(561, np.complex128)] * na,
-                    [np.zeros(3072)] * na, gs, as_, 1, O.DP)
-        ref.execute(O.MAP2ALM, sp, [np.zeros(561, np.complex128) for _ in range(na)],
-                    [np.zeros(3072)] * na, gs, as_, 1, O.DP)
-    first = None
-    dt, done = 0.0, 0
-    while done < nreal and dt < min_seconds:
-        T0, E0, B0 = realisation(done)
-        alm_t, alm_eb = [T0], [E0, B0]
-        map_t = [np.zeros(npix)]
-        map_qu = [np.zeros(npix), np.zeros(npix)]
-        out_t = [np.zeros_like(T0)]
-        out_eb = [np.zeros_like(E0), np.zeros_like(B0)]
+    for nt in ((1, nreal) if batched else (1,)):
+        for sp, na in ((0, 1), (2, 2)):
+            ref.execute(O.ALM2MAP, sp, [np.zeros(561, np.complex128) for _ in range(na * nt)],
+                        [np.zeros(3072) for _ in range(na * nt)], gs, as_, nt, O.DP)
+            ref.execute(O.MAP2ALM, sp, [np.zeros(561, np.complex128) for _ in range(na * nt)],
+                        [np.zeros(3072) for _ in range(na * nt)], gs, as_, nt, O.DP)
+
+    def round_trip(alm_t, alm_eb, nt):
+        map_t = [np.zeros(npix) for _ in range(nt)]
+        map_qu = [np.zeros(npix) for _ in range(2 * nt)]
+        out_t = [np.zeros_like(alm_t[0]) for _ in range(nt)]
+        out_eb = [np.zeros_like(alm_t[0]) for _ in range(2 * nt)]
         t0 = time.perf_counter()
-        ref.execute(O.ALM2MAP, 0, alm_t, map_t, g, a, 1, O.DP)
-        ref.execute(O.ALM2MAP, 2, alm_eb, map_qu, g, a, 1, O.DP)
-        ref.execute(O.MAP2ALM, 0, out_t, map_t, g, a, 1, O.DP)
-        ref.execute(O.MAP2ALM, 2, out_eb, map_qu, g, a, 1, O.DP)
-        dt += time.perf_counter() - t0
+        ops = ref.execute(O.ALM2MAP, 0, alm_t, map_t, g, a, nt, O.DP)[1]
+        ops += ref.execute(O.ALM2MAP, 2, alm_eb, map_qu, g, a, nt, O.DP)[1]
+        ops += ref.execute(O.MAP2ALM, 0, out_t, map_t, g, a, nt, O.DP)[1]
+        ops += ref.execute(O.MAP2ALM, 2, out_eb, map_qu, g, a, nt, O.DP)[1]
+        return time.perf_counter() - t0, ops, (map_t, map_qu, out_t, out_eb)
+
+    first, times, ops1 = None, [], 0
+    while len(times) < min(2, nreal) or (sum(times) < min_seconds and len(times) < nreal):
+        T0, E0, B0 = realisation(len(times))
+        dt, ops1, outs = round_trip([T0], [E0, B0], 1)
+        times.append(dt)
         if first is None:
+            map_t, map_qu, out_t, out_eb = outs
             first = ((map_t[0], map_qu[0], map_qu[1]), (out_t[0], out_eb[0], out_eb[1]))
-        done += 1
-        log("  cpu: realisation %d done, %.1f s so far" % (done - 1, dt))
-    info = {"value": done / dt, "unit": "TQU round trips/s", "cores": threads,
+        log("  cpu: realisation %d, ntrans 1: %.2f s" % (len(times) - 1, dt))
+    best = min(times)
+    info = {"value": 1.0 / best, "unit": "TQU round trips/s", "cores": threads,
             "kind": "reference",
-            "sample": "%d TQU realisation(s) of the batch of %d: libsharp alm2map spin "
-                      "0+2 then map2alm spin 0+2 at Nside %d lmax %d, %.1f s"
-                      % (done, nreal, nside, lmax, dt)}
+            "gflops": ops1 / best / 1e9,
+            "cpu_model": model, "physical_cores_on_host": phys, "threads": threads,
+            "sample": "libsharp alm2map spin 0+2 then map2alm spin 0+2 of one realisation "
+                      "(ntrans 1) at Nside %d lmax %d, best of %d tries (%.1f s), "
+                      "libsharp's own protocol" % (nside, lmax, len(times), sum(times))}
+    if batched:
+        rs = [realisation(i) for i in range(nreal)]
+        dt, ops, _ = round_trip([r[0] for r in rs], [x for r in rs for x in (r[1], r[2])], nreal)
+        log("  cpu: %d realisations as ntrans %d: %.2f s" % (nreal, nreal, dt))
+        info["batched_ntrans%d" % nreal] = {
+            "value": nreal / dt, "unit": "TQU round trips/s", "gflops": ops / dt / 1e9,
+            "sample": "all %d realisations in one call per spin and direction (ntrans %d, "
+                      "recurrence shared), one try (%.1f s)" % (nreal, nreal, dt)}
     return info, first[0], first[1]
 
 
@@ -147,48 +194,46 @@ def measured_traffic(workload, stage):
 
 def roofline(stats, mixed, nt, workload=None, bf16=False):
     """Roofline object of the slowest stage.  Legendre stages carry libsharp's
-    flop model per launch (c0 + c1*ntrans per triple: c0 the FP64 recurrence,
-    c1*ntrans the contraction); ring stages carry compulsory HBM bytes.
-    `traffic` is the stage's measured HBM bytes per launch."""
+    flop model per launch (c0 + c1*ntrans per (m, ring pair, l) triple,
+    libsharp/sharp_core_inc2.c:171-173,440-442); `achieved` is that
+    algorithmic figure over the average launch time, priced at the peak of
+    the unit the contraction runs on (BF16, FP32 or FP64 matrix cores).  Ring
+    stages carry compulsory HBM bytes.  `traffic` is the stage's measured HBM
+    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, kept in
+    profiles/hbm_traffic.json)."""
     dom = max(stats, key=lambda x: x["ms"])
     avg_s = dom["ms"] * 1e-3 / dom["launches"]
     traffic = measured_traffic(workload, dom["name"]) if workload else None
+    common = {"kernel": dom["name"], "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
+              "traffic_source": "profiles/hbm_traffic.json" if traffic else None}
     if dom["flops"] > 0:
         ach = dom["flops"] / dom["launches"] / avg_s
         if mixed:
             c = contraction_share(dom["name"], nt)
-            ach_c = ach * c
+            peak = PEAK["bf16"] if bf16 else PEAK["fp32"]
+            out = dict(common, bound="mfma", achieved=ach / 1e12, peak=peak / 1e12,
+                       unit="TFLOP/s", frac=ach / peak,
+                       model_contraction_tflops=ach * c / 1e12,
+                       fp64_recurrence_tflops=ach * (1 - c) / 1e12)
             if bf16:
-                # six BF16 products per FP32 product: the matrix cores run 6x
-                # the model's contraction flops, priced at the BF16 peak
-                return {"kernel": dom["name"], "bound": "mfma", "achieved": 6 * ach_c / 1e12,
-                        "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
-                        "frac": 6 * ach_c / PEAK["bf16"], "traffic": traffic,
-                        "avg_launch_ms": avg_s * 1e3,
-                        "model_contraction_tflops": ach_c / 1e12,
-                        "fp64_recurrence_tflops": ach * (1 - c) / 1e12,
-                        "note": "achieved = 6 x the contraction flops of libsharp's flop "
-                                "model (six BF16 piece products per FP32 product) over "
-                                "the dense BF16 MFMA peak; the FP64 recurrence share "
-                                "(c0 per triple) is given separately"}
-            return {"kernel": dom["name"], "bound": "mfma", "achieved": ach_c / 1e12,
-                    "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                    "frac": ach_c / PEAK["fp32"], "traffic": traffic,
-                    "avg_launch_ms": avg_s * 1e3,
-                    "fp64_recurrence_tflops": ach * (1 - c) / 1e12,
-                    "note": "achieved = contraction flops of libsharp's flop model "
-                            "(c1*ntrans per triple) over the FP32 MFMA peak; the FP64 "
-                            "recurrence share (c0 per triple) is given separately"}
-        return {"kernel": dom["name"], "bound": "mfma", "achieved": ach / 1e12,
-                "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-                "frac": ach / PEAK["fp64"], "traffic": traffic,
-                "avg_launch_ms": avg_s * 1e3,
-                "note": "FP64 work priced at the MI355X FP64 peak (vector and matrix "
-                        "FP64 peaks are equal); flops = libsharp's flop model"}
+                # the matrix cores run six BF16 piece products per FP32
+                # product of the contraction: their issue rate, for reference
+                out["bf16_mfma_issue_tflops"] = 6 * ach * c / 1e12
+                out["bf16_mfma_issue_frac"] = 6 * ach * c / PEAK["bf16"]
+                out["note"] = ("achieved = libsharp's flop model (whole launch) over the "
+                               "dense BF16 MFMA peak; bf16_mfma_issue_* = the six BF16 "
+                               "products per contraction product the matrix cores execute")
+            else:
+                out["note"] = ("achieved = libsharp's flop model (whole launch) over the "
+                               "FP32 MFMA peak")
+            return out
+        return dict(common, bound="mfma", achieved=ach / 1e12, peak=PEAK["fp64"] / 1e12,
+                    unit="TFLOP/s", frac=ach / PEAK["fp64"],
+                    note="libsharp's flop model over the MI355X FP64 peak (vector and "
+                         "matrix FP64 peaks are equal)")
     ach = dom["bytes"] / dom["launches"] / avg_s
-    return {"kernel": dom["name"], "bound": "hbm", "achieved": ach / 1e9,
-            "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / PEAK["hbm"],
-            "traffic": traffic, "avg_launch_ms": avg_s * 1e3}
+    return dict(common, bound="hbm", achieved=ach / 1e9, peak=PEAK["hbm"] / 1e9,
+                unit="GB/s", frac=ach / PEAK["hbm"])
 
 
 def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
@@ -382,6 +427,8 @@ def main():
                     help="time only --form (no comparison run)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-batched", action="store_true",
+                    help="skip the reference's ntrans=batch CPU figure")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates HBM counters)")
     ap.add_argument("--streams", type=int, default=1, choices=[1, 2],
@@ -507,7 +554,8 @@ def main():
         log("cpu baseline (reference libsharp, one realisation)")
         cpu, rmaps, ralms = cpu_baseline(
             nside, lmax, lambda i: (T[i].cpu().numpy(), E[i].cpu().numpy(),
-                                    B[i].cpu().numpy()), batch)
+                                    B[i].cpu().numpy()), batch,
+            batched=not args.no_cpu_batched)
         if cpu is not None:
             fm, fa = res["fp64"]["maps"], res["fp64"]["alms"]
             gm = (fm[0].cpu().numpy(), fm[batch].cpu().numpy(), fm[batch + 1].cpu().numpy())
@@ -531,7 +579,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"mixed": "f32-mfma contraction, f64 recurrence/FFT",
                   "bf16x3": "bf16-mfma contraction on 3-piece splits of f32 (6 products, "
-                            "f32 accumulation), f64 recurrence/FFT",
+                            "f32 accumulation; the spin-0 synthesis and the exponent-"
+                            "scaled spin-2 synthesis blocks on f32 mfma), f64 "
+                            "recurrence/FFT",
                   "fp64": "f64"}[args.form],
         "data": "synthetic: seeded Gaussian a_lm, C_l^TT=1/(l(l+1)+1) x 5' beam, "
                 "EE=0.1TT, BB=0.01TT, TE corr 0.5",
