@@ -79,14 +79,16 @@ template <int C> __device__ inline int xred_index(int lane)
 // spin 0 synthesis
 // ===========================================================================
 template <int TG>
-__global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a)
+__global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a, int nsb)
 {
   const int lane = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int m = blockIdx.y * MB + w;
+  int mb, sb;
+  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
+  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax;
-  const int slot = blockIdx.x * 64 + lane;
+  const int slot = sb * 64 + lane;
   const bool valid = slot < a.nslots;
   const int s = valid ? slot : a.nslots - 1;
   const double cth = a.cth[s], sth = a.sth[s];
@@ -238,15 +240,17 @@ __device__ inline void saddstep(QU &px, QU &py, double rxp, double rxm,
 }
 
 template <int TG, bool DERIV>
-__global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a)
+__global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
 {
   constexpr int NALM = DERIV ? 1 : 2;
   const int lane = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int m = blockIdx.y * MB + w;
+  int mb, sb;
+  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
+  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax, sp = a.spin;
-  const int slot = blockIdx.x * 64 + lane;
+  const int slot = sb * 64 + lane;
   const bool valid = slot < a.nslots;
   const int s = valid ? slot : a.nslots - 1;
   const double cth = a.cth[s], sth = a.sth[s];
@@ -675,25 +679,28 @@ static void check_launch(const char *what)
 
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st)
 {
-  const dim3 grid((a.nslots + 63) / 64, (a.mmax + MB) / MB);
+  // the slot blocks of one m block (which read the same coefficient and a_lm
+  // rows) on one XCD
+  const int nsb = (a.nslots + 63) / 64;
+  const dim3 grid(xcd_grid(a.nmb, nsb));
   const dim3 block(64, MB);
   if (a.spin == 0) {
     switch (ntg) {
-    case 1: hipLaunchKernelGGL(k_synth_spin0<1>, grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_synth_spin0<2>, grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_synth_spin0<4>, grid, block, 0, st, a); break;
+    case 1: hipLaunchKernelGGL(k_synth_spin0<1>, grid, block, 0, st, a, nsb); break;
+    case 2: hipLaunchKernelGGL(k_synth_spin0<2>, grid, block, 0, st, a, nsb); break;
+    case 4: hipLaunchKernelGGL(k_synth_spin0<4>, grid, block, 0, st, a, nsb); break;
     default: SHP_FAIL("bad transform group");
     }
   } else if (a.deriv1) {
     switch (ntg) {
-    case 1: hipLaunchKernelGGL((k_synth_spin<1, true>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_synth_spin<2, true>), grid, block, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_synth_spin<1, true>), grid, block, 0, st, a, nsb); break;
+    case 2: hipLaunchKernelGGL((k_synth_spin<2, true>), grid, block, 0, st, a, nsb); break;
     default: SHP_FAIL("bad transform group");
     }
   } else {
     switch (ntg) {
-    case 1: hipLaunchKernelGGL((k_synth_spin<1, false>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_synth_spin<2, false>), grid, block, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_synth_spin<1, false>), grid, block, 0, st, a, nsb); break;
+    case 2: hipLaunchKernelGGL((k_synth_spin<2, false>), grid, block, 0, st, a, nsb); break;
     default: SHP_FAIL("bad transform group");
     }
   }

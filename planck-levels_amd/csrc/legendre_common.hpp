@@ -9,6 +9,23 @@
 
 namespace shp {
 
+// Maps a linear workgroup id to (major, minor) so that the nminor workgroups
+// of one major index land on the same XCD (the hardware deals workgroup ids
+// round-robin over the 8 XCDs, each with its own L2): workgroups that read
+// the same rows share one L2.  False for the padding ids of the last round.
+__device__ inline bool xcd_map(int id, int nmajor, int nminor, int &major, int &minor)
+{
+  const int xcd = id & 7, k = id >> 3;
+  major = (k / nminor) * 8 + xcd;
+  minor = k % nminor;
+  return major < nmajor;
+}
+
+static inline int xcd_grid(int nmajor, int nminor)
+{
+  return ((nmajor + 7) / 8) * 8 * nminor;
+}
+
 // Brings |val| into [maxval*2^-800, maxval].  A finite double needs at most
 // three 800-binade steps; the bound also ends the loop for non-finite input.
 __device__ inline void normalize(double &val, int &scale, double maxval)

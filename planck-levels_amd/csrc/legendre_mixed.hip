@@ -135,22 +135,6 @@ __device__ inline f4 mfma_bf3(const Bf3 &a, const Bf3 &b, f4 d)
   d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, d, 0, 0, 0);
   return d;
 }
-// Maps a linear workgroup id to (major, minor) so that the nminor workgroups
-// of one major index land on the same XCD (ids are dealt round-robin over 8).
-// False for the padding ids of the last round.
-__device__ inline bool xcd_map(int id, int nmajor, int nminor, int &major, int &minor)
-{
-  const int xcd = id & 7, k = id >> 3;
-  major = (k / nminor) * 8 + xcd;
-  minor = k % nminor;
-  return major < nmajor;
-}
-
-static inline int xcd_grid(int nmajor, int nminor)
-{
-  return ((nmajor + 7) / 8) * 8 * nminor;
-}
-
 // Analysis: as xcd_map, but the 8 m of one phase block (m / 8: a phase cache
 // line holds the 8 m of a ring, phase_index) go to one XCD, issued next to
 // each other for each ring group, so that they share the block's lines in
