@@ -342,10 +342,12 @@ def batch_workload(S, torch, dev, args):
     """The uneven batch: 24 spin-0 maps each at Nside 128, 256, 512, 1024
     (lmax = 2 Nside) and 8 Gauss-Legendre grids each at lmax 255 and 767
     (nlat = lmax+1, nlon = 2 lmax+1); a_lm with C_l = 1/(l(l+1)+1).  One step
-    is alm2map + map2alm of all 112 maps: the mixed form as two batches (one
-    persistent Legendre launch each, replayed HIP graphs), the FP64 form one
-    call per geometry beside it.  The error of every mixed map and round-trip
-    a_lm against the FP64 form is reported."""
+    is alm2map + map2alm of all 112 maps.  Each form runs as two batches (one
+    persistent Legendre launch per direction, tile shape tuned at creation,
+    replayed HIP graphs): BF16-split matrix cores (`value`), FP32 matrix
+    cores and FP64; the FP64 form one call per geometry is timed beside them.
+    The error of every map and round-trip a_lm against the FP64 per-geometry
+    form is reported."""
     specs = [("healpix", n, 2 * n, 24) for n in (128, 256, 512, 1024)]
     specs += [("gauss", 255, 255, 8), ("gauss", 767, 767, 8)]
     gen = torch.Generator(device=dev).manual_seed(args.seed)
@@ -357,58 +359,64 @@ def batch_workload(S, torch, dev, args):
         npix = geom.npix
         jobs.append({"name": "%s %d lmax %d x%d" % (kind, n, lmax, nt), "geom": geom,
                      "ainfo": ainfo, "nt": nt, "alms": alms,
-                     "maps": [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(nt)],
-                     "out": [torch.zeros_like(alms[0]) for _ in range(nt)],
                      "maps64": [torch.zeros(npix, dtype=torch.float64, device=dev) for _ in range(nt)],
                      "out64": [torch.zeros_like(alms[0]) for _ in range(nt)]})
     nmaps = sum(j["nt"] for j in jobs)
-    flags = S.DP | S.MIXED_FP32
-    log("batch: creating plans (autotuning, graph capture)")
-    bs = S.Batch([(S.ALM2MAP, 0, j["alms"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
-                 flags, use_graph=True)
-    ba = S.Batch([(S.MAP2ALM, 0, j["out"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
-                 flags, use_graph=True)
-
-    def step_mixed():
-        bs.execute()
-        ba.execute()
 
     def step_fp64():
         for j in jobs:
             S.execute_device(S.ALM2MAP, 0, j["alms"], j["maps64"], j["geom"], j["ainfo"], j["nt"], S.DP)
             S.execute_device(S.MAP2ALM, 0, j["out64"], j["maps64"], j["geom"], j["ainfo"], j["nt"], S.DP)
 
-    dt, stats = run_form(S, torch, step_mixed, args.warmup, args.steps)
-    ms = 1e3 * dt / args.steps
-    show("batch mixed", stats, ms, nmaps / (dt / args.steps))
     dt64, stats64 = run_form(S, torch, step_fp64, args.warmup, args.steps)
     ms64 = 1e3 * dt64 / args.steps
     show("per-geometry fp64", stats64, ms64, nmaps / (dt64 / args.steps))
 
     def rel(x, y):
         return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
-    per = {}
-    for j in jobs:
-        per[j["name"]] = {
-            "maps_max_rel_l2": max(rel(a, b) for a, b in zip(j["maps"], j["maps64"])),
-            "roundtrip_alm_max_rel_l2": max(rel(a, b) for a, b in zip(j["out"], j["out64"]))}
-    log("per-map error vs FP64:", per)
+    res = {}
+    for form, flag in (("bf16x3", S.MIXED_BF16X3), ("fp32", S.MIXED_FP32), ("fp64", 0)):
+        for j in jobs:
+            j["maps"] = [torch.zeros_like(x) for x in j["maps64"]]
+            j["out"] = [torch.zeros_like(x) for x in j["out64"]]
+        log("batch %s: creating plans (autotuning, graph capture)" % form)
+        bs = S.Batch([(S.ALM2MAP, 0, j["alms"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
+                     S.DP | flag, use_graph=True)
+        ba = S.Batch([(S.MAP2ALM, 0, j["out"], j["maps"], j["geom"], j["ainfo"], j["nt"]) for j in jobs],
+                     S.DP | flag, use_graph=True)
+
+        def step():
+            bs.execute()
+            ba.execute()
+
+        dt, stats = run_form(S, torch, step, args.warmup, args.steps)
+        ms = 1e3 * dt / args.steps
+        show("batch " + form, stats, ms, nmaps / (dt / args.steps))
+        per = {}
+        for j in jobs:
+            per[j["name"]] = {
+                "maps_max_rel_l2": max(rel(a, b) for a, b in zip(j["maps"], j["maps64"])),
+                "roundtrip_alm_max_rel_l2": max(rel(a, b) for a, b in zip(j["out"], j["out64"]))}
+        res[form] = {"value": nmaps / (dt / args.steps), "ms_per_step": ms,
+                     "tiles": {"synthesis": bs.config(), "analysis": ba.config()},
+                     "error_vs_fp64_per_geometry": per}
+        log("batch %s: per-map error vs FP64 per geometry:" % form, per)
+        bs.close()
+        ba.close()
     out = {
-        "metric": "uneven batch alm2map+map2alm, maps/s (mixed batched form)",
-        "value": nmaps / (dt / args.steps), "unit": "map round trips/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "metric": "uneven batch alm2map+map2alm, maps/s (BF16-split batched form)",
+        "value": res["bf16x3"]["value"], "unit": "map round trips/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["bf16x3"]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-mfma contraction, f64 recurrence/FFT",
+        "dtype": "bf16-mfma contraction on 3-piece splits of f32 (spin-0 synthesis on f32 "
+                 "mfma), f64 recurrence/FFT",
         "data": "synthetic: seeded Gaussian a_lm, C_l=1/(l(l+1)+1)",
         "config": {"workload": "96 HEALPix maps (24 each Nside 128/256/512/1024, lmax=2 Nside) "
-                               "+ 16 Gauss-Legendre grids (8 each lmax 255/767), spin 0",
-                   "batch_config": {"synthesis": bs.config(), "analysis": ba.config()}},
+                               "+ 16 Gauss-Legendre grids (8 each lmax 255/767), spin 0"},
+        "forms": res,
         "fp64_per_geometry": {"value": nmaps / (dt64 / args.steps), "ms_per_step": ms64},
-        "error_vs_fp64_per_geometry": per,
     }
     print(json.dumps(out), flush=True)
-    bs.close()
-    ba.close()
 
 
 def main():

@@ -32,7 +32,7 @@ extern "C" {
    every FP32 operand is split exactly into three BF16 pieces and the six
    piece products of order >= 2^-16 are accumulated in FP32 (the dropped ones
    are below 2^-20 of each product, about 2^-24 in the rms); outputs stay
-   within 1e-6 relative L2 of the FP64 form.  Not for batches. */
+   within 1e-6 relative L2 of the FP64 form. */
 #define SHARP_HIP_MIXED_BF16X3  (1<<25)
 
 /* Device-resident form of sharp_execute (libsharp/sharp_lowlevel.h:222-224).
@@ -78,15 +78,21 @@ void sharp_hip_map2alm_iter_device (int spin, void *alm, void *map,
    sharp_execute per geometry).  `alm`/`map` are host arrays of device
    pointers as in sharp_execute.  All jobs of a batch are syntheses
    (SHARP_ALM2MAP, SHARP_WY, SHARP_ALM2MAP_DERIV1) or all analyses
-   (SHARP_MAP2ALM, SHARP_Yt), with the same `flags`.  With
-   SHARP_HIP_MIXED_FP32 the Legendre stage of all jobs runs as one persistent
-   launch per kernel kind (spin 0, spin s, deriv1), with the transform group
-   size and the workgroups per CU chosen by timing the batch at creation
+   (SHARP_MAP2ALM, SHARP_Yt), with the same `flags`: FP64, or one of the
+   mixed forms (SHARP_HIP_MIXED_FP32, SHARP_HIP_MIXED_BF16X3).  The Legendre
+   stage of all jobs runs as one persistent launch per kernel kind (spin 0,
+   spin s, deriv1) over a work list of tiles of every job, dealt to the XCDs
+   by cost.  The tile shape -- transforms per tile, dispatch (persistent
+   workgroups per CU taking tiles from per-XCD counters, or, for syntheses,
+   one flat launch), orders m per synthesis workgroup (8 or 4, one per
+   wavefront of 64 ring pairs) or ring pairs per analysis workgroup (256 or
+   128) -- is chosen by timing the batch at creation
    (cached by batch shape; SHARP_HIP_TUNE=0 disables, SHARP_HIP_TUNE_CACHE
    names a file that keeps the choices).  With use_graph the whole batch is
    recorded into a HIP graph at creation and each execute replays it: the
    device buffers of the jobs must stay the same (their contents may
-   change). */
+   change).  Executes of one batch must not overlap in time (they share its
+   work buffers and counters); different batches may run concurrently. */
 typedef struct {
   int type;                         /* a sharp_jobtype */
   int spin;
@@ -104,9 +110,15 @@ sharp_hip_batch *sharp_hip_batch_create (int njobs, const sharp_hip_job *jobs,
 /* Enqueues the batch on `stream` (NULL: the null stream). */
 void sharp_hip_batch_execute (sharp_hip_batch *batch, void *stream);
 /* The tuned configuration: transforms per tile, persistent workgroups per
-   CU, and whether a graph is replayed. */
+   CU (0: the synthesis runs as one flat launch with a wavefront per tile),
+   and whether a graph is replayed. */
 void sharp_hip_batch_config (const sharp_hip_batch *batch, int *tgroup,
   int *wg_per_cu, int *graph);
+/* The tuned tile shape: orders m per synthesis workgroup (0 for analysis
+   batches) and wavefronts of 64 ring pairs per analysis workgroup (0 for
+   synthesis batches). */
+void sharp_hip_batch_tiles (const sharp_hip_batch *batch, int *mblock,
+  int *ring_waves);
 void sharp_hip_batch_destroy (sharp_hip_batch *batch);
 
 /* Builds (and caches) the device tables for this geometry / a_lm layout /

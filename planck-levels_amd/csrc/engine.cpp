@@ -1241,23 +1241,42 @@ static void map_extent(const sharp_geom_info *g, int64_t &lo, int64_t &hi)
 // ---------------------------------------------------------------------------
 struct BatchWork {
   int kind = 0, ncb = 1, grid = 0, nitems = 0;
+  int flat_grid = 0;    // workgroups of the flat synthesis launch
+  int tg = 0;           // transforms of every item (FP64 syntheses), else 0
   BatchItem *items = nullptr;
   int *xofs = nullptr;
+  int *ctr = nullptr;   // 8 item counters, one per XCD part of the list
   double flops = 0.;
   const char *name = "";
 };
 
+// The tile shape of a batch, chosen by batch_tune.
+struct BatchTiles {
+  int tgroup = 0;      // transforms per tile: the form's maximum >> tgroup
+  int wg_per_cu = 2;   // persistent workgroups per CU
+  int flat = 0;        // syntheses: 1 = one flat launch (a wavefront per tile)
+                       // instead of persistent workgroups
+  int shape = 0;       // wavefronts per workgroup: synthesis m-blocks of 8 (0) or 4
+                       // (1) orders m, one per wavefront, of 64 ring pairs;
+                       // analysis ring block of 4 (0) or 2 (1) wavefronts of 64
+                       // ring pairs
+};
+
 struct BatchPlan {
   int device = 0;
-  bool synth = true, mixed = true;
+  bool synth = true;
+  int contr = kContrF64;   // Legendre contraction: FP64, FP32 or BF16-split matrix cores
   std::vector<Job> jobs;
   std::vector<std::vector<void *>> almp, mapp;
-  std::vector<DevBuf> phase, almtmp, part;
+  std::vector<DevBuf> phase, almtmp, part, bsplit, gmax2;
   std::vector<JobState> st;
-  LegArgs *djobs = nullptr;
+  // per-job launch arguments for each analysis ring block (4, 2 wavefronts):
+  // their ring groups and group maxima of mlim differ
+  LegArgs *djobs[2] = {nullptr, nullptr};
+  std::vector<int> ngroups[2];
+  std::vector<int *> dgmax[2];
   std::vector<BatchWork> work;
-  int tgroup_small = 1;   // tuned: 0 -> small transform groups, 1 -> large
-  int wg_per_cu = 2;      // tuned: persistent workgroups per CU
+  BatchTiles tiles;
   bool use_graph = true;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -1265,19 +1284,18 @@ struct BatchPlan {
 
   void free_work()
   {
-    for (auto &w : work) { (void)hipFree(w.items); (void)hipFree(w.xofs); }
+    for (auto &w : work) { (void)hipFree(w.items); (void)hipFree(w.xofs); (void)hipFree(w.ctr); }
     work.clear();
   }
   ~BatchPlan()
   {
     free_work();
-    (void)hipFree(djobs);
+    for (auto *d : djobs) (void)hipFree(d);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
-    for (auto &b : phase) b.release();
-    for (auto &b : almtmp) b.release();
-    for (auto &b : part) b.release();
+    for (auto *v : {&phase, &almtmp, &part, &bsplit, &gmax2})
+      for (auto &b : *v) b.release();
   }
 };
 
@@ -1298,19 +1316,43 @@ static const char *batch_name(int kind)
   }
 }
 
-// Work lists for the tuning choice (transform group size, workgroups per
-// CU): tiles grouped by (job, transform group, m-block or m) so that a
-// group's tiles run on one XCD; groups are dealt to the 8 XCDs largest
-// first onto the least loaded one, and each XCD's list is ordered by cost.
+// largest transform group of a kernel kind in a form: the mixed forms take
+// up to 32 real MFMA columns; the FP64 matrix-core analysis 16; the FP64
+// VALU syntheses 4 (spin 0) or 2 (spin s) transforms per recurrence
+static int batch_tmax(int kind, int contr)
+{
+  const bool spin0 = (kind == kBatchSynth0 || kind == kBatchAnal0);
+  const bool synth = kind <= kBatchSynthD;
+  if (contr != kContrF64) return spin0 ? kMixMaxT0 : kMixMaxTS;
+  if (synth) return spin0 ? 4 : 2;
+  return spin0 ? kF64MaxT0 : kF64MaxTS;
+}
+
+// analysis wavefronts per workgroup of a ring-block choice
+static int batch_waves(int shape) { return shape ? 2 : 4; }
+
+template <typename G> static size_t nitems_kind(const std::vector<G> &groups)
+{
+  size_t n = 0;
+  for (auto &g : groups) n += g.items.size();
+  return n;
+}
+
+// Work lists for a tile shape: tiles grouped by (job, transform group,
+// m-block or m) so that a group's tiles run on one XCD; groups are dealt to
+// the 8 XCDs largest first onto the least loaded one.
 static void build_work(BatchPlan &b, int ncu)
 {
   b.free_work();
+  const BatchTiles &tl = b.tiles;
   for (int kind = kBatchSynth0; kind <= kBatchAnalS; ++kind) {
     const bool spin0 = (kind == kBatchSynth0 || kind == kBatchAnal0);
     const bool synth = kind <= kBatchSynthD;
-    const int tg = spin0 ? (b.tgroup_small ? kMixMaxT0 / 2 : kMixMaxT0)
-                         : (b.tgroup_small ? kMixMaxTS / 2 : kMixMaxTS);
-    struct Group { double cost; std::vector<BatchItem> items; };
+    const int tg = std::max(1, batch_tmax(kind, b.contr) >> tl.tgroup);
+    const int wi = synth ? 0 : tl.shape;
+    // tl: the transforms of the group's items when the kind's kernel is
+    // specialised by it (FP64 syntheses), else 0
+    struct Group { double cost; std::vector<BatchItem> items; int tl = 0; };
     std::vector<Group> groups;
     double flops = 0.;
     for (size_t ji = 0; ji < b.jobs.size(); ++ji) {
@@ -1319,90 +1361,139 @@ static void build_work(BatchPlan &b, int ncu)
       const JobState &s = b.st[ji];
       flops += s.triples * (s.c0 + s.c1 * j.ntrans);
       const int nslots = s.gp->nslots;
-      for (int t0 = 0; t0 < j.ntrans; t0 += tg) {
-        const int ntl = std::min(tg, j.ntrans - t0);
+      int t0 = 0;
+      while (t0 < j.ntrans) {
+        int ntl = std::min(tg, j.ntrans - t0);
+        // the FP64 syntheses have kernels for power-of-two groups only
+        if (synth && b.contr == kContrF64)
+          while (ntl & (ntl - 1)) ntl &= ntl - 1;
         if (synth) {
+          // the 8 orders m of a phase block (whose phases share cache lines)
+          // in one group, tiles of `waves` orders m x 64 slots
+          const int waves = 2 * batch_waves(tl.shape), per = MB / waves;
           const int nmb = (j.mmax + MB) / MB, nsb = (nslots + 63) / 64;
           for (int mb = 0; mb < nmb; ++mb) {
             Group g;
-            g.cost = (double)(j.lmax - mb * MB + 1) * nsb * (s.c0 + s.c1 * ntl);
+            g.tl = b.contr == kContrF64 ? ntl : 0;
+            g.cost = (double)(j.lmax - mb * MB + 1) * MB * nsb * (s.c0 + s.c1 * ntl);
             for (int sb = 0; sb < nsb; ++sb)
-              g.items.push_back(BatchItem{(int)ji, t0, ntl, 0, mb, sb, 0, 0});
+              for (int h = 0; h < per; ++h)
+                g.items.push_back(BatchItem{(int)ji, t0, ntl, 0, mb * per + h, sb, 0, 0});
             groups.push_back(std::move(g));
           }
         } else {
-          const auto &gm = s.ml->hgmax;
+          const int gs = 64 * batch_waves(wi);
+          const int ng = (nslots + gs - 1) / gs;
+          std::vector<int> gm(ng, -1);
+          for (int k = 0; k < nslots; ++k) gm[k / gs] = std::max(gm[k / gs], s.ml->h[k]);
           for (int m = 0; m <= j.mmax; ++m) {
             Group g;
-            for (int gi = 0; gi < (int)gm.size(); ++gi)
+            for (int gi = 0; gi < ng; ++gi)
               if (gm[gi] >= m)
                 g.items.push_back(BatchItem{(int)ji, t0, ntl, 2 * t0 * j.nalm, m, gi, 0, 0});
             if (g.items.empty()) continue;
-            g.cost = (double)(j.lmax - m + 1) * g.items.size() * (s.c0 + s.c1 * ntl);
+            g.cost = (double)(j.lmax - m + 1) * g.items.size() * gs * (s.c0 + s.c1 * ntl);
             groups.push_back(std::move(g));
           }
         }
+        t0 += ntl;
       }
     }
     if (groups.empty()) continue;
     std::stable_sort(groups.begin(), groups.end(),
                      [](const Group &x, const Group &y) { return x.cost > y.cost; });
-    std::vector<BatchItem> lists[8];
-    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (auto &g : groups) {
-      const int x = (int)(std::min_element(load, load + 8) - load);
-      load[x] += g.cost;
-      lists[x].insert(lists[x].end(), g.items.begin(), g.items.end());
+    std::vector<int> tls;
+    for (auto &g : groups)
+      if (std::find(tls.begin(), tls.end(), g.tl) == tls.end()) tls.push_back(g.tl);
+    for (int tlv : tls) {
+      std::vector<BatchItem> lists[8];
+      double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (auto &g : groups) {
+        if (g.tl != tlv) continue;
+        const int x = (int)(std::min_element(load, load + 8) - load);
+        load[x] += g.cost;
+        lists[x].insert(lists[x].end(), g.items.begin(), g.items.end());
+      }
+      std::vector<BatchItem> all;
+      std::vector<int> xofs(9, 0);
+      for (int x = 0; x < 8; ++x) {
+        xofs[x] = (int)all.size();
+        all.insert(all.end(), lists[x].begin(), lists[x].end());
+      }
+      xofs[8] = (int)all.size();
+      int maxntl = 1;
+      for (const BatchItem &it : all) maxntl = std::max(maxntl, it.ntl);
+      BatchWork w;
+      w.kind = kind;
+      w.tg = tlv;
+      w.ncb = (b.contr != kContrF64 && maxntl * (spin0 ? 2 : 4) > 16) ? 2 : 1;
+      w.nitems = (int)all.size();
+      w.grid = std::max(8, ((tl.wg_per_cu * ncu + 7) / 8) * 8);
+      if (synth) {
+        // flat launch: a workgroup per item, 8 parts
+        int most = 0;
+        for (int x = 0; x < 8; ++x) most = std::max(most, xofs[x + 1] - xofs[x]);
+        w.flat_grid = 8 * most;
+      }
+      w.items = upload(all);
+      w.xofs = upload(xofs);
+      w.ctr = upload(std::vector<int>(8, 0));
+      w.flops = flops * (double)all.size() / std::max<size_t>(1, nitems_kind(groups));
+      w.name = batch_name(kind);
+      b.work.push_back(w);
     }
-    std::vector<BatchItem> all;
-    std::vector<int> xofs(9, 0);
-    for (int x = 0; x < 8; ++x) {
-      xofs[x] = (int)all.size();
-      all.insert(all.end(), lists[x].begin(), lists[x].end());
-    }
-    xofs[8] = (int)all.size();
-    BatchWork w;
-    w.kind = kind;
-    w.ncb = (tg * (spin0 ? 2 : 4) > 16) ? 2 : 1;
-    w.nitems = (int)all.size();
-    w.grid = std::max(8, ((b.wg_per_cu * ncu + 7) / 8) * 8);
-    w.items = upload(all);
-    w.xofs = upload(xofs);
-    w.flops = flops;
-    w.name = batch_name(kind);
-    b.work.push_back(w);
   }
+}
+
+// the tile shape's launch arguments in every job's scatter
+static void batch_scatter_args(BatchPlan &b)
+{
+  const int wi = b.tiles.shape;
+  for (size_t i = 0; i < b.st.size(); ++i) {
+    JobState &s = b.st[i];
+    s.aa.gmax = b.dgmax[wi][i];
+    s.aa.ngroups = b.ngroups[wi][i];
+  }
+}
+
+static BatchLaunch batch_launch(const BatchPlan &b, const BatchWork &w, int shape)
+{
+  BatchLaunch l;
+  l.kind = w.kind;
+  l.ncb = w.ncb;
+  l.contr = b.contr;
+  l.shape = shape;
+  l.tg = w.tg;
+  l.flat = b.synth && b.tiles.flat;
+  l.grid = w.grid;
+  l.flat_grid = w.flat_grid;
+  l.items = w.items;
+  l.xofs = w.xofs;
+  l.ctr = w.ctr;
+  return l;
 }
 
 static void batch_run(Ctx &c, BatchPlan &b, hipStream_t st)
 {
   const size_t n = b.jobs.size();
-  if (!b.mixed) {
-    // FP64: the jobs one after another
+  const int shape = batch_waves(b.tiles.shape) * (b.synth ? 2 : 1);
+  const LegArgs *dj = b.djobs[b.synth ? 0 : b.tiles.shape];
+  if (b.synth) {
     for (size_t i = 0; i < n; ++i) {
       JobState &s = b.st[i];
-      if (b.synth) {
+      if (s.aa.bsplit)
+        timed(c, st, "alm_gather", 0., s.alm_bytes, [&] { launch_alm_gather_split(s.aa, st); });
+      else
         stage_gather(c, s, st);
-        stage_legendre_fp64(c, s, st);
-        stage_phase_to_rings(c, s, st);
-      } else {
-        stage_rings_to_phase(c, s, st);
-        stage_analysis(c, s, b.part[i].get(part_bytes(s.j, (int)s.ml->hgmax.size())), st);
-      }
     }
-    return;
-  }
-  if (b.synth) {
-    for (size_t i = 0; i < n; ++i) stage_gather(c, b.st[i], st);
     for (auto &w : b.work)
-      timed(c, st, w.name, w.flops, 0.,
-            [&] { launch_batch(w.kind, w.ncb, b.djobs, w.items, w.xofs, w.grid, st); });
+      timed(c, st, w.name, w.flops, 0., [&] { launch_batch(batch_launch(b, w, shape), dj, st); });
     for (size_t i = 0; i < n; ++i) stage_phase_to_rings(c, b.st[i], st);
   } else {
     for (size_t i = 0; i < n; ++i) stage_rings_to_phase(c, b.st[i], st);
     for (auto &w : b.work)
-      timed(c, st, w.name, w.flops, 0.,
-            [&] { launch_batch(w.kind, w.ncb, b.djobs, w.items, w.xofs, w.grid, st); });
+      timed(c, st, w.name, w.flops, 0., [&] { launch_batch(batch_launch(b, w, shape), dj, st); });
+    batch_scatter_args(b);
     for (size_t i = 0; i < n; ++i) {
       JobState &s = b.st[i];
       timed(c, st, "alm_scatter", 0., s.alm_bytes, [&] { launch_alm_scatter(s.aa, st); });
@@ -1414,6 +1505,7 @@ static void batch_run(Ctx &c, BatchPlan &b, hipStream_t st)
 static std::string batch_key(const BatchPlan &b)
 {
   std::string k = b.synth ? "S" : "A";
+  k += std::to_string(b.contr);
   char buf[96];
   for (const Job &j : b.jobs) {
     const GeomPlan &gp = *b.st[&j - &b.jobs[0]].gp;
@@ -1425,7 +1517,7 @@ static std::string batch_key(const BatchPlan &b)
   return k;
 }
 
-static std::map<std::string, std::pair<int, int>> g_tune;   // key -> (tgroup_small, wg_per_cu)
+static std::map<std::string, BatchTiles> g_tune;
 
 static const char *tune_file()
 {
@@ -1433,7 +1525,8 @@ static const char *tune_file()
   return (f && f[0]) ? f : nullptr;
 }
 
-static bool tune_lookup(const std::string &key, std::pair<int, int> &v)
+// cache file lines: "<key> <tgroup> <wg_per_cu> <shape> <flat>"
+static bool tune_lookup(const std::string &key, BatchTiles &v)
 {
   auto it = g_tune.find(key);
   if (it != g_tune.end()) { v = it->second; return true; }
@@ -1444,40 +1537,47 @@ static bool tune_lookup(const std::string &key, std::pair<int, int> &v)
   std::vector<char> line(1 << 16);
   bool found = false;
   while (fgets(line.data(), (int)line.size(), fp)) {
-    int a = 0, w = 0;
-    char *sp = strrchr(line.data(), ' ');
-    if (!sp) continue;
-    *sp = 0;
-    char *sp2 = strrchr(line.data(), ' ');
-    if (!sp2) continue;
-    *sp2 = 0;
-    a = atoi(sp2 + 1);
-    w = atoi(sp + 1);
-    if (key == line.data()) { v = {a, w}; found = true; }
+    char *kend = strchr(line.data(), ' ');
+    if (!kend) continue;
+    *kend = 0;
+    BatchTiles t;
+    if (sscanf(kend + 1, "%d %d %d %d", &t.tgroup, &t.wg_per_cu, &t.shape, &t.flat) != 4)
+      continue;
+    if (key == line.data() && t.tgroup >= 0 && t.tgroup <= 2 && t.wg_per_cu >= 1 &&
+        t.wg_per_cu <= 8 && (t.shape == 0 || t.shape == 1) && (t.flat == 0 || t.flat == 1)) {
+      v = t;
+      found = true;
+    }
   }
   fclose(fp);
   if (found) g_tune[key] = v;
   return found;
 }
 
-static void tune_store(const std::string &key, std::pair<int, int> v)
+static void tune_store(const std::string &key, const BatchTiles &v)
 {
   g_tune[key] = v;
   const char *f = tune_file();
   if (!f) return;
   if (FILE *fp = fopen(f, "a")) {
-    fprintf(fp, "%s %d %d\n", key.c_str(), v.first, v.second);
+    fprintf(fp, "%s %d %d %d %d\n", key.c_str(), v.tgroup, v.wg_per_cu, v.shape, v.flat);
     fclose(fp);
   }
 }
 
-// Project autotuner: times every (transform group, workgroups per CU) pair
-// on the batch itself (one untimed and one timed run each, HIP events on
-// the plan's stream) and keeps the fastest; the choice is cached by the
-// batch's shape (and in SHARP_HIP_TUNE_CACHE, when set).
+// Project autotuner: times every tile shape -- transform group (the form's
+// maximum, half, quarter) x dispatch (persistent workgroups per CU 1, 2, 4,
+// fewer where fewer stay resident; syntheses also one flat launch with a
+// workgroup per tile) x workgroup shape (synthesis: m-blocks of 8 or 4
+// orders m, a wavefront of 64 ring pairs each; analysis: ring blocks of 4 or
+// 2 wavefronts of 64 ring pairs) -- on the batch itself (one untimed and one
+// timed run each, HIP events on the plan's stream) and keeps the fastest;
+// the choice is cached by the batch's shape (and in SHARP_HIP_TUNE_CACHE,
+// when set).  Shapes that give the same work lists as an earlier one (a
+// transform group larger than every job's count) are skipped.
 static void batch_tune(Ctx &c, BatchPlan &b, int ncu)
 {
-  std::pair<int, int> best{0, 2};
+  BatchTiles best;
   const std::string key = batch_key(b);
   const char *e = std::getenv("SHARP_HIP_TUNE");
   const bool off = e && e[0] == '0';
@@ -1486,26 +1586,36 @@ static void batch_tune(Ctx &c, BatchPlan &b, int ncu)
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    for (int small = 0; small <= 1; ++small)
-      for (int wpc : {1, 2, 4}) {
-        b.tgroup_small = small;
-        b.wg_per_cu = wpc;
-        build_work(b, ncu);
-        batch_run(c, b, b.cap);
-        HIP_CHECK(hipEventRecord(e0, b.cap));
-        batch_run(c, b, b.cap);
-        HIP_CHECK(hipEventRecord(e1, b.cap));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < tbest) { tbest = ms; best = {small, wpc}; }
-      }
+    // a transform group no smaller than every job's count gives the same
+    // work lists as the larger one before it
+    auto same_lists = [&](int tgroup) {
+      for (const Job &j : b.jobs)
+        if (j.ntrans > std::max(1, batch_tmax(batch_kind(j), b.contr) >> tgroup)) return false;
+      return true;
+    };
+    for (int tgroup = 0; tgroup <= 2; ++tgroup) {
+      if (tgroup > 0 && same_lists(tgroup)) continue;
+      // dispatch: persistent workgroups per CU 1, 2, 4; syntheses also flat
+      for (int wpc : {1, 2, 4, 0})
+        for (int shape = 0; shape <= 1; ++shape) {
+          if (wpc == 0 && !b.synth) continue;
+          b.tiles = BatchTiles{tgroup, wpc ? wpc : 2, shape, wpc == 0 ? 1 : 0};
+          build_work(b, ncu);
+          batch_run(c, b, b.cap);
+          HIP_CHECK(hipEventRecord(e0, b.cap));
+          batch_run(c, b, b.cap);
+          HIP_CHECK(hipEventRecord(e1, b.cap));
+          HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < tbest) { tbest = ms; best = b.tiles; }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tune_store(key, best);
   }
-  b.tgroup_small = best.first;
-  b.wg_per_cu = best.second;
+  b.tiles = best;
   build_work(b, ncu);
 }
 
@@ -1767,9 +1877,7 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
   Ctx &c = context();
   auto b = std::make_unique<BatchPlan>();
   b->device = c.device;
-  b->mixed = is_mixed(flags);
-  if (flags & SHARP_HIP_MIXED_BF16X3)
-    SHP_FAIL("batches run the FP32 mixed form (SHARP_HIP_MIXED_FP32) or FP64");
+  b->contr = is_mixed(flags) ? mixed_contr(flags) : kContrF64;
   b->use_graph = use_graph != 0;
   for (int i = 0; i < njobs; ++i) {
     const sharp_hip_job &q = jobs[i];
@@ -1784,9 +1892,11 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
     b->mapp.emplace_back(mp, mp + (size_t)j.ntrans * j.nmaps);
   }
   const size_t n = b->jobs.size();
-  b->phase.resize(n);
-  b->almtmp.resize(n);
-  b->part.resize(n);
+  for (auto *v : {&b->phase, &b->almtmp, &b->part, &b->bsplit, &b->gmax2}) v->resize(n);
+  for (int wi = 0; wi < 2; ++wi) {
+    b->ngroups[wi].resize(n);
+    b->dgmax[wi].resize(n);
+  }
   for (size_t i = 0; i < n; ++i) {
     const Job &j = b->jobs[i];
     GeomPlan &gp = get_geom(c, j.g);
@@ -1795,30 +1905,62 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
     b->st.push_back(setup_job(c, j, b->almp[i].data(), b->mapp[i].data(), 0, 0, ph, at, nullptr,
                               c.stream));
     JobState &s = b->st.back();
-    if (b->mixed && !b->synth) {
-      // partial sums of every transform of the job: [group][row][2*ncomp_alm]
+    // ring groups of the two analysis ring blocks (4 and 2 wavefronts) and
+    // the largest m limit of each
+    b->ngroups[0][i] = (int)s.ml->hgmax.size();
+    b->dgmax[0][i] = s.ml->gmax;
+    const int gs2 = 128, ng2 = (gp.nslots + gs2 - 1) / gs2;
+    std::vector<int> hg2(ng2, -1);
+    for (int k = 0; k < gp.nslots; ++k) hg2[k / gs2] = std::max(hg2[k / gs2], s.ml->h[k]);
+    int *d2 = static_cast<int *>(b->gmax2[i].get(hg2.size() * sizeof(int)));
+    HIP_CHECK(hipMemcpy(d2, hg2.data(), hg2.size() * sizeof(int), hipMemcpyHostToDevice));
+    b->ngroups[1][i] = ng2;
+    b->dgmax[1][i] = d2;
+    if (!b->synth) {
+      // partial sums of every transform of the job, [group][row][2*ncomp_alm],
+      // with room for the groups of the smaller ring block
       const int64_t nrows = tri_ofs(j.mmax + 1, j.lmax);
-      const size_t ng = s.ml->hgmax.size();
-      s.la.part = b->part[i].get(ng * (size_t)nrows * 2 * s.ncomp_alm * sizeof(float));
+      const size_t esz = b->contr == kContrF64 ? sizeof(double) : sizeof(float);
+      s.la.part = b->part[i].get((size_t)std::max(ng2, b->ngroups[0][i]) * (size_t)nrows * 2 *
+                                 s.ncomp_alm * esz);
       s.la.part_ld = 2 * s.ncomp_alm;
       s.aa.part = s.la.part;
-      s.aa.part_f64 = 0;
+      s.aa.part_f64 = b->contr == kContrF64 ? 1 : 0;
       s.aa.part_ld = s.la.part_ld;
       s.aa.part_c0 = 0;
       s.aa.c0 = 0;
       s.aa.npc = s.ncomp_alm;
     }
+    if (uses_bsplit(j)) {
+      // the BF16 spin-s synthesis reads the a_lm split into BF16 pieces by
+      // the gather
+      const int ncol = bs_ncol(j);
+      BsPlan &bp = get_bsplan(c, j.lmax, j.mmax, j.spin, ncol);
+      s.keep.push_back(owner(c.bsplans, [&](const BsPlan &x) { return &x == &bp; }));
+      unsigned *bs = static_cast<unsigned *>(b->bsplit[i].get((size_t)bp.total * sizeof(unsigned)));
+      s.aa.bsplit = bs;
+      s.la.bsplit = bs;
+      s.aa.bs_base = s.la.bs_base = bp.d;
+      s.aa.bs_ncol = s.la.bs_ncol = ncol;
+    }
   }
-  std::vector<LegArgs> las;
-  for (auto &s : b->st) las.push_back(s.la);
-  b->djobs = upload(las);
+  for (int wi = 0; wi < 2; ++wi) {
+    std::vector<LegArgs> las;
+    for (size_t i = 0; i < n; ++i) {
+      LegArgs la = b->st[i].la;
+      la.gmax = b->dgmax[wi][i];
+      la.ngroups = b->ngroups[wi][i];
+      las.push_back(la);
+    }
+    b->djobs[wi] = upload(las);
+  }
   HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
   int ncu = 256;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess)
     ncu = 256;
   const bool prof = c.prof.on;
   c.prof.on = false;
-  if (b->mixed) batch_tune(c, *b, ncu);
+  batch_tune(c, *b, ncu);
   batch_run(c, *b, b->cap);   // settles every lazily built plan before capture
   HIP_CHECK(hipStreamSynchronize(b->cap));
   if (b->use_graph) {
@@ -1851,13 +1993,20 @@ void sharp_hip_batch_config(const sharp_hip_batch *batch, int *tgroup, int *wg_p
 {
   const BatchPlan *b = reinterpret_cast<const BatchPlan *>(batch);
   if (!b) SHP_FAIL("null batch");
-  if (tgroup) {
-    const bool spin0 = !b->jobs.empty() && b->jobs[0].spin == 0;
-    const int big = spin0 ? kMixMaxT0 : kMixMaxTS;
-    *tgroup = b->mixed ? (b->tgroup_small ? big / 2 : big) : 0;
-  }
-  if (wg_per_cu) *wg_per_cu = b->mixed ? b->wg_per_cu : 0;
+  if (tgroup)
+    *tgroup = b->jobs.empty() ? 0
+                              : std::max(1, batch_tmax(batch_kind(b->jobs[0]), b->contr) >>
+                                                b->tiles.tgroup);
+  if (wg_per_cu) *wg_per_cu = (b->synth && b->tiles.flat) ? 0 : b->tiles.wg_per_cu;
   if (graph) *graph = b->exec ? 1 : 0;
+}
+
+void sharp_hip_batch_tiles(const sharp_hip_batch *batch, int *mblock, int *ring_waves)
+{
+  const BatchPlan *b = reinterpret_cast<const BatchPlan *>(batch);
+  if (!b) SHP_FAIL("null batch");
+  if (mblock) *mblock = b->synth ? 2 * batch_waves(b->tiles.shape) : 0;
+  if (ring_waves) *ring_waves = b->synth ? 0 : batch_waves(b->tiles.shape);
 }
 
 void sharp_hip_batch_destroy(sharp_hip_batch *batch)

@@ -63,10 +63,32 @@ struct BatchItem {
   int major, minor, pad0, pad1;
 };
 enum { kBatchSynth0, kBatchSynthS, kBatchSynthD, kBatchAnal0, kBatchAnalS };
-// grid: number of persistent workgroups (a multiple of 8); xofs[0..8]: the
-// work list's parts for the 8 XCDs
-void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items,
-                  const int *xofs, int grid, hipStream_t st);
+// One launch of a batched Legendre kernel kind over its work list.
+struct BatchLaunch {
+  int kind;
+  int ncb;          // 16-column blocks of the mixed forms (1 or 2)
+  int contr;        // kContr*
+  int shape;        // wavefronts per workgroup: synthesis 4 or 8 (items are
+                    // (m-block of `shape` orders m, slot block) tiles, one order
+                    // m per wavefront), analysis 2 or 4 (items are (m, ring group
+                    // of 64*shape slots) tiles, whose largest m limits are in the
+                    // job's gmax)
+  int tg;           // transforms of every item (FP64 syntheses), else 0
+  bool flat;        // syntheses: one item per workgroup, grid flat_grid;
+                    // else persistent workgroups taking items from counters
+  int grid;         // persistent workgroups (a multiple of 8; capped at what
+                    // stays resident)
+  int flat_grid;    // flat workgroups: 8 x the largest part
+  const BatchItem *items;
+  const int *xofs;  // [9]: the work list's parts for the 8 XCDs
+  int *ctr;         // [8]: the parts' item counters (zeroed by the launch)
+};
+void launch_batch(const BatchLaunch &l, const LegArgs *jobs, hipStream_t st);
+// the FP64 syntheses of a batch (legendre.hip): every item of the list has
+// tg transforms, 1, 2 or 4 (spin 0) / 1 or 2 (spin s)
+void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArgs *jobs,
+                            const BatchItem *items, const int *xofs, int *ctr, int grid,
+                            int flat_grid, hipStream_t st);
 
 // ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st);

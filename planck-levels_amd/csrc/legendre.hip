@@ -30,6 +30,7 @@
 // (libsharp/sharp_core_inc2.c:521-619).
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "kernels.hpp"
 #include "legendre_common.hpp"
 
@@ -78,14 +79,15 @@ template <int C> __device__ inline int xred_index(int lane)
 // ===========================================================================
 // spin 0 synthesis
 // ===========================================================================
-template <int TG>
-__global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a, int nsb)
+// One wavefront: order m (wave-uniform) for the 64 ring pairs of slot block
+// sb, TG transforms from a.t0.
+// CROW: rows through the constant address space (the batch kernels, whose
+// job arguments come from memory; kernel-argument pointers get scalar loads
+// without it, and plain pointers schedule slightly better)
+template <int TG, bool CROW = false>
+__device__ void synth_spin0_tile(const LegArgs &a, int m, int sb)
 {
   const int lane = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  int mb, sb;
-  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
-  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax;
   const int slot = sb * 64 + lane;
@@ -99,10 +101,10 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a, int nsb)
   for (int t = 0; t < TG; ++t) p1r[t] = p1i[t] = p2r[t] = p2i[t] = 0.;
 
   if (wave_any(active)) {
-    const double2 *rf =
-        reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+    typedef std::conditional_t<CROW, URow2, const double2 *> Row2;
+    const Row2 rf = Row2(reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m));
     const int nc = a.ncomp_alm;
-    const double2 *al = a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0);
+    const Row2 al = Row2(a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0));
 
     double lam1 = 0., lam2;
     int scale;
@@ -239,15 +241,11 @@ __device__ inline void saddstep(QU &px, QU &py, double rxp, double rxm,
   py.ui = fma(-ag.x, lx, py.ui);
 }
 
-template <int TG, bool DERIV>
-__global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
+template <int TG, bool DERIV, bool CROW = false>
+__device__ void synth_spin_tile(const LegArgs &a, int m, int sb)
 {
   constexpr int NALM = DERIV ? 1 : 2;
   const int lane = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  int mb, sb;
-  if (!xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) return;
-  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax, sp = a.spin;
   const int slot = sb * 64 + lane;
@@ -265,10 +263,12 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
   }
 
   if (wave_any(active) && st.mhi <= lmax) {
-    const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+    typedef std::conditional_t<CROW, URow, const double *> Row;
+    typedef std::conditional_t<CROW, URow2, const double2 *> Row2;
+    const Row fxb = Row(a.tab + 3 * (tab_ofs(m, lmax) - m));
     auto fx = [&](int l) { return Fx{fxb[3 * l], fxb[3 * l + 1], fxb[3 * l + 2]}; };
     const int nc = a.ncomp_alm;
-    const double2 *al = a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0 * NALM);
+    const Row2 al = Row2(a.alm + ((tri_ofs(m, lmax) - m) * nc + a.t0 * NALM));
     auto ag = [&](int l, int t) { return al[(int64_t)l * nc + t * NALM]; };
     auto ac = [&](int l, int t) {
       return DERIV ? make_double2(0., 0.) : al[(int64_t)l * nc + t * NALM + 1];
@@ -370,6 +370,68 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
       a.phase[phase_index(cq, m, slot, 1, a.nslots, a.nmb)] = qs;
       a.phase[phase_index(cu, m, slot, 1, a.nslots, a.nmb)] = us;
     }
+  }
+}
+
+// ===========================================================================
+// synthesis kernels: one job (grid over its m-blocks and slot blocks, the
+// slot blocks of one m-block, which read the same coefficient and a_lm rows,
+// on one XCD), or a batch (persistent grid over a work list)
+// ===========================================================================
+__device__ inline int wave_y() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
+template <int TG>
+__global__ void __launch_bounds__(64 * MB) k_synth_spin0(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) synth_spin0_tile<TG>(a, mb * MB + wave_y(), sb);
+}
+
+template <int TG, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
+    synth_spin_tile<TG, DERIV>(a, mb * MB + wave_y(), sb);
+}
+
+// items: (m-block of blockDim.y orders m, slot block) tiles, one order m per
+// wavefront (so that a workgroup writes whole phase lines), all of TG
+// transforms (one kernel per group size: a kernel holding every size needs
+// the registers of the largest and keeps half as many wavefronts resident);
+// FLAT: one item per workgroup, else persistent
+template <int TG, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_synth_spin0(const LegArgs *__restrict__ jobs,
+                                                          const BatchItem *__restrict__ items,
+                                                          const int *__restrict__ xofs,
+                                                          int *__restrict__ ctr)
+{
+  auto tile = [&](const BatchItem &it) {
+    synth_spin0_tile<TG, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(), it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
+  }
+}
+
+template <int TG, bool DERIV, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_synth_spin(const LegArgs *__restrict__ jobs,
+                                                         const BatchItem *__restrict__ items,
+                                                         const int *__restrict__ xofs,
+                                                         int *__restrict__ ctr)
+{
+  auto tile = [&](const BatchItem &it) {
+    synth_spin_tile<TG, DERIV, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(),
+                                     it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
   }
 }
 
@@ -705,6 +767,35 @@ void launch_synth(const LegArgs &a, int ntg, hipStream_t st)
     }
   }
   check_launch("synthesis");
+}
+
+template <bool FLAT>
+static void batch_synth_f64(int kind, int tg, dim3 b, int grid, int flat_grid, hipStream_t st,
+                            const LegArgs *jobs, const BatchItem *items, const int *xofs,
+                            int *ctr)
+{
+  auto go = [&](auto kernel) {
+    launch_batch_kernel(kernel, FLAT, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  };
+  if (kind == kBatchSynth0) {
+    if (tg == 4) go(kb_synth_spin0<4, FLAT>);
+    else if (tg == 2) go(kb_synth_spin0<2, FLAT>);
+    else go(kb_synth_spin0<1, FLAT>);
+  } else if (kind == kBatchSynthS) {
+    if (tg == 2) go(kb_synth_spin<2, false, FLAT>); else go(kb_synth_spin<1, false, FLAT>);
+  } else {
+    if (tg == 2) go(kb_synth_spin<2, true, FLAT>); else go(kb_synth_spin<1, true, FLAT>);
+  }
+}
+
+void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArgs *jobs,
+                            const BatchItem *items, const int *xofs, int *ctr, int grid,
+                            int flat_grid, hipStream_t st)
+{
+  const dim3 b(64, shape);
+  if (flat) batch_synth_f64<true>(kind, tg, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  else batch_synth_f64<false>(kind, tg, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  check_launch("batched FP64 synthesis");
 }
 
 void launch_anal(const LegArgs &a, int ntg, hipStream_t st)

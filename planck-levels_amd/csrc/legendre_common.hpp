@@ -5,7 +5,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "shp_internal.hpp"
+#include "kernels.hpp"
 
 namespace shp {
 
@@ -24,6 +26,131 @@ __device__ inline bool xcd_map(int id, int nmajor, int nminor, int &major, int &
 static inline int xcd_grid(int nmajor, int nminor)
 {
   return ((nmajor + 7) / 8) * 8 * nminor;
+}
+
+// Wave-uniform rows (recurrence coefficients, canonical a_lm) read through
+// the constant address space.  The rows do not change during a kernel; with
+// that known the compiler turns a wave-uniform address into a scalar load
+// even where the row pointer itself was read from memory (the job arguments
+// of a batch) -- for a pointer that is a kernel argument it finds this out by
+// itself -- and the values stay in scalar registers.
+typedef const __attribute__((address_space(4))) double *ConstRowPtr;
+struct URow {
+  ConstRowPtr q;
+  __device__ explicit URow(const double *p) : q((ConstRowPtr)p) {}
+  __device__ double operator[](int64_t i) const { return q[i]; }
+};
+struct URow2 {
+  ConstRowPtr q;
+  __device__ explicit URow2(const double2 *p) : q((ConstRowPtr)p) {}
+  __device__ double2 operator[](int64_t i) const { return make_double2(q[2 * i], q[2 * i + 1]); }
+};
+
+// Wave-uniform copy of a struct read from memory: each 32-bit word through
+// readfirstlane, so that the compiler keeps the fields in scalar registers
+// (as it does for kernel arguments) instead of vector registers, where the
+// job arguments of a batch tile cost about 40 registers per lane and with
+// them resident wavefronts.
+template <typename T> __device__ inline T uniform_load(const T *p)
+{
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  T v;
+  const int *src = reinterpret_cast<const int *>(p);
+  int *dst = reinterpret_cast<int *>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return v;
+}
+
+// Batch kernels.  The work list is split into 8 contiguous parts, xofs[0..8],
+// one per XCD (the host groups the items that share rows into one part);
+// workgroups with blockIdx.x % 8 == x, which the hardware places on XCD x,
+// take the items of part x.  Every item is a tile of one workgroup.
+//
+// Flat form: workgroup b takes item b/8 of part b%8 (grid: 8 x the largest
+// part); the hardware deals the workgroups as resources free up.
+template <typename F>
+__device__ inline void for_flat_item(const BatchItem *items, const int *xofs, F &&f)
+{
+  const int x = blockIdx.x & 7, p = xofs[x] + (blockIdx.x >> 3);
+  if (p < xofs[x + 1]) f(uniform_load(items + p));
+}
+
+// Persistent form: each workgroup takes items of its part from the counter
+// ctr[x] (zeroed before the launch) until the part is used up, so that
+// workgroups that start late, or are slow, simply take fewer items.  The
+// first thread fetches, the workgroup follows.
+template <typename F>
+__device__ inline void for_group_items(const BatchItem *items, const int *xofs, int *ctr,
+                                       int &fetched, F &&f)
+{
+  const int x = blockIdx.x & 7, lo = xofs[x], n = xofs[x + 1] - lo;
+  for (;;) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) fetched = atomicAdd(&ctr[x], 1);
+    __syncthreads();
+    const int p = fetched;
+    __syncthreads();   // every thread has read `fetched` before the next fetch
+    if (p >= n) break;
+    f(uniform_load(items + lo + p));
+    __syncthreads();   // the next item reuses the LDS of this one
+  }
+}
+
+// zeroes the 8 per-XCD item counters of a persistent launch (a kernel, not
+// a memset, so that graphs replay it correctly)
+template <int N> __global__ void k_zero_ints(int *p)
+{
+  if (threadIdx.x < N) p[threadIdx.x] = 0;
+}
+
+// Launches a batch kernel: flat (grid flat_grid, no counters) or persistent
+// (below).
+template <typename K>
+static inline void launch_batch_kernel(K kernel, bool flat, dim3 block, int grid, int flat_grid,
+                                       hipStream_t st, const LegArgs *jobs,
+                                       const BatchItem *items, const int *xofs, int *ctr);
+
+// Launches a persistent batch kernel: its counters zeroed first on the same
+// stream, and its grid (a multiple of 8) no larger than the workgroups the
+// device keeps resident at once (more would only find the work taken).
+template <typename K>
+static inline void launch_persistent(K kernel, dim3 block, int grid, hipStream_t st,
+                                     const LegArgs *jobs, const BatchItem *items,
+                                     const int *xofs, int *ctr)
+{
+  hipLaunchKernelGGL(k_zero_ints<8>, dim3(1), dim3(64), 0, st, ctr);
+  int nb = 0, dev = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block.x * block.y * block.z, 0) !=
+          hipSuccess || nb < 1)
+    nb = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 256;
+  const int cap = std::max(8, (nb * ncu / 8) * 8);
+  hipLaunchKernelGGL(kernel, dim3(std::min(grid, cap)), block, 0, st, jobs, items, xofs, ctr);
+}
+
+template <typename K>
+static inline void launch_batch_kernel(K kernel, bool flat, dim3 block, int grid, int flat_grid,
+                                       hipStream_t st, const LegArgs *jobs,
+                                       const BatchItem *items, const int *xofs, int *ctr)
+{
+  if (flat) {
+    if (flat_grid > 0)
+      hipLaunchKernelGGL(kernel, dim3(flat_grid), block, 0, st, jobs, items, xofs, ctr);
+  } else {
+    launch_persistent(kernel, block, grid, st, jobs, items, xofs, ctr);
+  }
+}
+
+// the launch arguments of an item's job, for its transform group
+__device__ inline LegArgs job_args(const LegArgs *jobs, const BatchItem &it)
+{
+  LegArgs a = uniform_load(jobs + it.job);
+  a.t0 = it.t0;
+  a.ntl = it.ntl;
+  a.part_col0 = it.col0;
+  return a;
 }
 
 // Brings |val| into [maxval*2^-800, maxval].  A finite double needs at most

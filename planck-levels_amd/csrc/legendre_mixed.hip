@@ -37,6 +37,7 @@
 // one L2 only.
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "kernels.hpp"
 #include "legendre_common.hpp"
 
@@ -205,12 +206,14 @@ __device__ inline void store_phases(const LegArgs &a, int m, int sb, int comp0, 
 // complete out of order and are waited for as a whole) after the first
 // two-step of a block, so that the wait for the current coefficients does
 // not also wait for the prefetch.
-template <int NCB>
-__device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
+// One wavefront: order m (wave-uniform) for the 64 ring pairs of slot block sb.
+// CROW: read the coefficient rows through the constant address space (the
+// batch kernels, whose job arguments come from memory; kernel-argument
+// pointers get scalar loads without it, and plain pointers schedule better)
+template <int NCB, bool CROW = false>
+__device__ void msynth_spin0_tile(const LegArgs &a, int m, int sb)
 {
   const int lane = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax, nc = a.ncomp_alm, ncol = 2 * a.ntl;
   const int slot = sb * 64 + lane;
@@ -227,8 +230,8 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
     for (int r = 0; r < 4; ++r) tot1[cb][r] = tot2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
 
   if (wave_any(active)) {
-    const double2 *rf =
-        reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
+    typedef std::conditional_t<CROW, URow2, const double2 *> Row2;
+    const Row2 rf = Row2(reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m));
     const int lpad = lmax + kTabPad;   // last readable entry of the row
     // row l of this m, first column of this launch
     const float *arow = reinterpret_cast<const float *>(a.alm) +
@@ -434,12 +437,10 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int mb, int sb)
 // ===========================================================================
 // CT: 0 FP32 matrix cores; 1 BF16 matrix cores on three-piece splits in the
 // unscaled part (groups of 32 l), FP32 for the scaled blocks and the tail
-template <int NCB, bool DERIV, int CT>
-__device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
+template <int NCB, bool DERIV, int CT, bool CROW = false>
+__device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
 {
   const int lane = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int m = mb * MB + w;
   if (m > a.mmax) return;
   const int lmax = a.lmax, nc = a.ncomp_alm, sp = a.spin, ncol = 4 * a.ntl;
   const int slot = sb * 64 + lane;
@@ -457,7 +458,8 @@ __device__ void msynth_spin_tile(const LegArgs &a, int mb, int sb)
     for (int r = 0; r < 4; ++r) tot1[cb][r] = tot2[cb][r] = f4{0.f, 0.f, 0.f, 0.f};
 
   if (wave_any(active) && st.mhi <= lmax) {
-    const double *fxb = a.tab + 3 * (tab_ofs(m, lmax) - m);
+    typedef std::conditional_t<CROW, URow, const double *> Row;
+    const Row fxb = Row(a.tab + 3 * (tab_ofs(m, lmax) - m));
     const int lpad = lmax + kTabPad;
     auto fx = [&](int l) {
       const int k = min(l, lpad);
@@ -891,13 +893,13 @@ __device__ inline void write_tiles(T *slice, const typename Mfma<T>::acc (&t)[2]
 
 // CT: 0 the matrix cores of T (FP32 or FP64); 1 (T = float) the BF16
 // matrix cores on exact three-piece splits of FP32 operands
-template <typename T, int NCB, int CT>
+// W wavefronts per workgroup: ring group g is slots g*64W .. g*64W+64W-1.
+template <typename T, int NCB, int CT, int W>
 __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
-                                 T (&lds)[kMixWaves0][kSlice], double (&cbuf)[2][64])
+                                 T (&lds)[W][kSlice], double (&cbuf)[2][64])
 {
   static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
-  constexpr int W = kMixWaves0;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;   // no ring pair of this group reaches m
   const int lane = threadIdx.x & 63;
@@ -1121,13 +1123,12 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   }
 }
 
-template <typename T, int NCB, int CT>
+template <typename T, int NCB, int CT, int W>
 __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
-                                T (&lds)[kMixWavesS][2][kSlice], double (&cbuf)[2][96])
+                                T (&lds)[W][2][kSlice], double (&cbuf)[2][96])
 {
   static_assert(CT == 0 || sizeof(T) == 4, "BF16 splits are of FP32 values");
   typedef Mfma<T> MM;
-  constexpr int W = kMixWavesS;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;
   const int lane = threadIdx.x & 63;
@@ -1407,18 +1408,23 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
 // kernels: one job (grid over its tiles), or a batch (persistent grid over a
 // work list of several jobs)
 // ===========================================================================
+// wave-uniform index of this wavefront in its workgroup (the synthesis
+// workgroups are blockDim.y wavefronts of 64 lanes, one m each)
+__device__ inline int wave_y() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
 template <int NCB>
 __global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
 {
   int mb, sb;
-  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin0_tile<NCB>(a, mb, sb);
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin0_tile<NCB>(a, mb * MB + wave_y(), sb);
 }
 
 template <int NCB, bool DERIV, int CT>
 __global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
 {
   int mb, sb;
-  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb)) msynth_spin_tile<NCB, DERIV, CT>(a, mb, sb);
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
+    msynth_spin_tile<NCB, DERIV, CT>(a, mb * MB + wave_y(), sb);
 }
 
 template <typename T, int NCB, int CT>
@@ -1428,7 +1434,7 @@ __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
   __shared__ double cbuf[2][64];
   int m, g;
   if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin0_tile<T, NCB, CT>(a, m, g, lds, cbuf);
+    manal_spin0_tile<T, NCB, CT, kMixWaves0>(a, m, g, lds, cbuf);
 }
 
 template <typename T, int NCB, int CT>
@@ -1438,73 +1444,71 @@ __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
   __shared__ double cbuf[2][96];
   int m, g;
   if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin_tile<T, NCB, CT>(a, m, g, lds, cbuf);
+    manal_spin_tile<T, NCB, CT, kMixWavesS>(a, m, g, lds, cbuf);
 }
 
-// Persistent form: workgroup b works through the items of XCD b%8 (the work
-// list is split into 8 contiguous parts, xofs[0..8]) with a stride of the
-// number of workgroups per XCD, so the tiles of one (job, m-block) stay in
-// one L2.  The grid size is a multiple of 8.
-template <typename F>
-__device__ inline void for_items(const BatchItem *items, const int *xofs, F &&f)
+// Batch forms (for_flat_item / for_group_items): synthesis items are
+// (m-block of blockDim.y orders m, slot block) tiles, one order m per
+// wavefront (a workgroup then writes whole phase lines); analysis items are
+// (m, ring group of W wavefronts) tiles.
+template <int NCB, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_msynth_spin0(const LegArgs *__restrict__ jobs,
+                                                           const BatchItem *__restrict__ items,
+                                                           const int *__restrict__ xofs,
+                                                           int *__restrict__ ctr)
 {
-  const int x = blockIdx.x & 7, nper = gridDim.x >> 3;
-  for (int p = xofs[x] + (blockIdx.x >> 3); p < xofs[x + 1]; p += nper) {
-    const BatchItem it = items[p];
-    f(it);
+  auto tile = [&](const BatchItem &it) {
+    msynth_spin0_tile<NCB, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(), it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
   }
 }
 
-__device__ inline LegArgs job_args(const LegArgs *jobs, const BatchItem &it)
+template <int NCB, bool DERIV, int CT, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *__restrict__ jobs,
+                                                          const BatchItem *__restrict__ items,
+                                                          const int *__restrict__ xofs,
+                                                          int *__restrict__ ctr)
 {
-  LegArgs a = jobs[it.job];
-  a.t0 = it.t0;
-  a.ntl = it.ntl;
-  a.part_col0 = it.col0;
-  return a;
+  auto tile = [&](const BatchItem &it) {
+    msynth_spin_tile<NCB, DERIV, CT, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(),
+                                           it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
+  }
 }
 
-template <int NCB>
-__global__ void __launch_bounds__(64 * MB) kb_msynth_spin0(const LegArgs *jobs,
-                                                           const BatchItem *items, const int *xofs)
+template <typename T, int NCB, int CT, int W>
+__global__ void __launch_bounds__(64 * W) kb_manal_spin0(const LegArgs *__restrict__ jobs,
+                                                         const BatchItem *__restrict__ items,
+                                                         const int *__restrict__ xofs, int *__restrict__ ctr)
 {
-  for_items(items, xofs, [&](const BatchItem &it) {
-    msynth_spin0_tile<NCB>(job_args(jobs, it), it.major, it.minor);
-  });
-}
-
-template <int NCB, bool DERIV>
-__global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *jobs,
-                                                          const BatchItem *items, const int *xofs)
-{
-  for_items(items, xofs, [&](const BatchItem &it) {
-    msynth_spin_tile<NCB, DERIV, 0>(job_args(jobs, it), it.major, it.minor);
-  });
-}
-
-template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWaves0) kb_manal_spin0(const LegArgs *jobs,
-                                                                  const BatchItem *items,
-                                                                  const int *xofs)
-{
-  __shared__ float lds[kMixWaves0][kSlice];
+  __shared__ __align__(16) T lds[W][kSlice];
   __shared__ double cbuf[2][64];
-  for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin0_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
-    __syncthreads();   // the next item reuses the LDS slices
+  __shared__ int fetched;
+  for_group_items(items, xofs, ctr, fetched, [&](const BatchItem &it) {
+    manal_spin0_tile<T, NCB, CT, W>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
   });
 }
 
-template <int NCB>
-__global__ void __launch_bounds__(64 * kMixWavesS) kb_manal_spin(const LegArgs *jobs,
-                                                                 const BatchItem *items,
-                                                                 const int *xofs)
+template <typename T, int NCB, int CT, int W>
+__global__ void __launch_bounds__(64 * W) kb_manal_spin(const LegArgs *__restrict__ jobs,
+                                                        const BatchItem *__restrict__ items,
+                                                        const int *__restrict__ xofs, int *__restrict__ ctr)
 {
-  __shared__ float lds[kMixWavesS][2][kSlice];
+  __shared__ __align__(16) T lds[W][2][kSlice];
   __shared__ double cbuf[2][96];
-  for_items(items, xofs, [&](const BatchItem &it) {
-    manal_spin_tile<float, NCB, 0>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
-    __syncthreads();
+  __shared__ int fetched;
+  for_group_items(items, xofs, ctr, fetched, [&](const BatchItem &it) {
+    manal_spin_tile<T, NCB, CT, W>(job_args(jobs, it), it.major, it.minor, lds, cbuf);
   });
 }
 
@@ -1555,35 +1559,74 @@ void launch_msynth(const LegArgs &a, int contr, hipStream_t st)
   check_launch("mixed synthesis");
 }
 
-void launch_batch(int kind, int ncb, const LegArgs *jobs, const BatchItem *items,
-                  const int *xofs, int grid, hipStream_t st)
+template <int W>
+static void launch_batch_anal(int kind, bool two, int contr, const LegArgs *jobs,
+                              const BatchItem *items, const int *xofs, int *ctr, int grid,
+                              hipStream_t st)
 {
-  if (grid <= 0) return;
-  if (grid % 8) SHP_FAIL("persistent grid must be a multiple of 8");
-  const dim3 g(grid);
-  const bool two = ncb == 2;
-  switch (kind) {
-  case kBatchSynth0:
-    if (two) hipLaunchKernelGGL(kb_msynth_spin0<2>, g, dim3(64, MB), 0, st, jobs, items, xofs);
-    else hipLaunchKernelGGL(kb_msynth_spin0<1>, g, dim3(64, MB), 0, st, jobs, items, xofs);
-    break;
-  case kBatchSynthS:
-    if (two) hipLaunchKernelGGL((kb_msynth_spin<2, false>), g, dim3(64, MB), 0, st, jobs, items, xofs);
-    else hipLaunchKernelGGL((kb_msynth_spin<1, false>), g, dim3(64, MB), 0, st, jobs, items, xofs);
-    break;
-  case kBatchSynthD:
-    if (two) hipLaunchKernelGGL((kb_msynth_spin<2, true>), g, dim3(64, MB), 0, st, jobs, items, xofs);
-    else hipLaunchKernelGGL((kb_msynth_spin<1, true>), g, dim3(64, MB), 0, st, jobs, items, xofs);
-    break;
-  case kBatchAnal0:
-    if (two) hipLaunchKernelGGL(kb_manal_spin0<2>, g, dim3(64 * kMixWaves0), 0, st, jobs, items, xofs);
-    else hipLaunchKernelGGL(kb_manal_spin0<1>, g, dim3(64 * kMixWaves0), 0, st, jobs, items, xofs);
-    break;
-  case kBatchAnalS:
-    if (two) hipLaunchKernelGGL(kb_manal_spin<2>, g, dim3(64 * kMixWavesS), 0, st, jobs, items, xofs);
-    else hipLaunchKernelGGL(kb_manal_spin<1>, g, dim3(64 * kMixWavesS), 0, st, jobs, items, xofs);
-    break;
-  default: SHP_FAIL("bad batch kernel");
+  const dim3 b(64 * W);
+  auto go = [&](auto kernel) { launch_persistent(kernel, b, grid, st, jobs, items, xofs, ctr); };
+  if (contr == kContrF64) {
+    if (kind == kBatchAnal0) go(kb_manal_spin0<double, 1, 0, W>);
+    else go(kb_manal_spin<double, 1, 0, W>);
+    return;
+  }
+  const bool bf = contr == kContrBf16x3;
+  if (kind == kBatchAnal0) {
+    if (bf) { if (two) go(kb_manal_spin0<float, 2, 1, W>); else go(kb_manal_spin0<float, 1, 1, W>); }
+    else { if (two) go(kb_manal_spin0<float, 2, 0, W>); else go(kb_manal_spin0<float, 1, 0, W>); }
+  } else {
+    if (bf) { if (two) go(kb_manal_spin<float, 2, 1, W>); else go(kb_manal_spin<float, 1, 1, W>); }
+    else { if (two) go(kb_manal_spin<float, 2, 0, W>); else go(kb_manal_spin<float, 1, 0, W>); }
+  }
+}
+
+template <bool FLAT>
+static void batch_msynth(int kind, bool two, bool bf, dim3 b, int grid, int flat_grid,
+                         hipStream_t st, const LegArgs *jobs, const BatchItem *items,
+                         const int *xofs, int *ctr)
+{
+  auto go = [&](auto kernel) {
+    launch_batch_kernel(kernel, FLAT, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  };
+  if (kind == kBatchSynth0) {
+    // spin 0 keeps the FP32 matrix cores in both mixed forms
+    if (two) go(kb_msynth_spin0<2, FLAT>); else go(kb_msynth_spin0<1, FLAT>);
+  } else if (kind == kBatchSynthS) {
+    if (bf) { if (two) go(kb_msynth_spin<2, false, 1, FLAT>); else go(kb_msynth_spin<1, false, 1, FLAT>); }
+    else { if (two) go(kb_msynth_spin<2, false, 0, FLAT>); else go(kb_msynth_spin<1, false, 0, FLAT>); }
+  } else {
+    if (bf) { if (two) go(kb_msynth_spin<2, true, 1, FLAT>); else go(kb_msynth_spin<1, true, 1, FLAT>); }
+    else { if (two) go(kb_msynth_spin<2, true, 0, FLAT>); else go(kb_msynth_spin<1, true, 0, FLAT>); }
+  }
+}
+
+void launch_batch(const BatchLaunch &L, const LegArgs *jobs, hipStream_t st)
+{
+  if (L.grid <= 0) return;
+  if (L.grid % 8) SHP_FAIL("persistent grid must be a multiple of 8");
+  const bool two = L.ncb == 2;
+  const int kind = L.kind, shape = L.shape;
+  const bool synth = kind == kBatchSynth0 || kind == kBatchSynthS || kind == kBatchSynthD;
+  if (synth) {
+    // shape: wavefronts per workgroup, one order m each
+    if (shape != 4 && shape != 8) SHP_FAIL("bad batch synthesis workgroup");
+    if (L.contr == kContrF64) {
+      launch_batch_synth_f64(kind, L.tg, shape, L.flat, jobs, L.items, L.xofs, L.ctr, L.grid,
+                             L.flat_grid, st);
+      return;
+    }
+    const dim3 b(64, shape);
+    const bool bf = L.contr == kContrBf16x3;
+    if (L.flat) batch_msynth<true>(kind, two, bf, b, L.grid, L.flat_grid, st, jobs, L.items, L.xofs, L.ctr);
+    else batch_msynth<false>(kind, two, bf, b, L.grid, L.flat_grid, st, jobs, L.items, L.xofs, L.ctr);
+  } else {
+    // shape: wavefronts (of 64 ring pairs) per workgroup
+    if (shape == 4)
+      launch_batch_anal<4>(kind, two, L.contr, jobs, L.items, L.xofs, L.ctr, L.grid, st);
+    else if (shape == 2)
+      launch_batch_anal<2>(kind, two, L.contr, jobs, L.items, L.xofs, L.ctr, L.grid, st);
+    else SHP_FAIL("bad batch ring block");
   }
   check_launch("batched Legendre stage");
 }

@@ -115,6 +115,7 @@ def _load():
     lib.sharp_hip_batch_execute.argtypes = [ct.c_void_p, ct.c_void_p]
     lib.sharp_hip_batch_config.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int),
                                            ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
+    lib.sharp_hip_batch_tiles.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
     lib.sharp_hip_batch_destroy.argtypes = [ct.c_void_p]
     lib.sharp_hip_prepare.argtypes = [G, A, ct.c_int]
     lib.sharp_hip_synchronize.argtypes = [ct.c_void_p]
@@ -413,8 +414,10 @@ class Batch:
     def config(self):
         t, w, g = ct.c_int(), ct.c_int(), ct.c_int()
         LIB.sharp_hip_batch_config(self.ptr, ct.byref(t), ct.byref(w), ct.byref(g))
+        mb, rw = ct.c_int(), ct.c_int()
+        LIB.sharp_hip_batch_tiles(self.ptr, ct.byref(mb), ct.byref(rw))
         return {"transforms_per_tile": t.value, "workgroups_per_cu": w.value,
-                "graph": bool(g.value)}
+                "graph": bool(g.value), "m_block": mb.value, "ring_waves": rw.value}
 
     def close(self):
         if getattr(self, "ptr", None) and LIB is not None:

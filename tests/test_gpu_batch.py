@@ -14,8 +14,8 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-FORMS = {"fp64": 0, "fp32": S.MIXED_FP32}
-TOL = {"fp64": 1e-12, "fp32": 1e-6}
+FORMS = {"fp64": 0, "fp32": S.MIXED_FP32, "bf16x3": S.MIXED_BF16X3}
+TOL = {"fp64": 1e-12, "fp32": 1e-6, "bf16x3": 1e-6}
 
 
 def _spec_job(dev, rng, spec, synth):
@@ -66,7 +66,10 @@ def _check_batch(orc, dev, specs, synth, form, graph, reps=3, after_create=None)
     rng = np.random.default_rng(3)
     jobs = [_spec_job(dev, rng, sp, synth) for sp in specs]
     b = S.Batch([j for j, _ in jobs], S.DP | FORMS[form], use_graph=graph)
-    assert b.config()["graph"] == graph
+    cfg = b.config()
+    assert cfg["graph"] == graph
+    assert (cfg["m_block"] in (4, 8) and cfg["ring_waves"] == 0) if synth else \
+        (cfg["m_block"] == 0 and cfg["ring_waves"] in (2, 4)), cfg
     if after_create is not None:
         after_create()
     for rep in range(reps):
@@ -98,8 +101,8 @@ def test_batch_one_geometry_several_spins(gpu, oracle, form, graph):
     """Analysis jobs of spins 0, 2 and 1 on one geometry (a Nside no other
     test uses, so its cache entry is new): each job's per-ring m limits stay
     valid while the later jobs add theirs."""
-    nside = {("fp64", True): 12, ("fp64", False): 14,
-             ("fp32", True): 18, ("fp32", False): 20}[(form, graph)]
+    nside = {("fp64", True): 12, ("fp64", False): 14, ("fp32", True): 18,
+             ("fp32", False): 20, ("bf16x3", True): 22, ("bf16x3", False): 26}[(form, graph)]
     specs = [("healpix", nside, 30, 30, 0, 2), ("healpix", nside, 30, 30, 2, 1),
              ("healpix", nside, 30, 30, 1, 1), ("healpix", nside, 26, 20, 2, 1)]
     _check_batch(oracle, gpu, specs, False, form, graph, reps=2)
@@ -110,6 +113,7 @@ def test_batch_survives_release(gpu, oracle):
     jobs and still computes the right answer afterwards."""
     _check_batch(oracle, gpu, UNEVEN[:2], False, "fp32", True, reps=2, after_create=S.release)
     _check_batch(oracle, gpu, UNEVEN[:2], True, "fp64", False, reps=2, after_create=S.release)
+    _check_batch(oracle, gpu, UNEVEN, True, "bf16x3", True, reps=2, after_create=S.release)
 
 
 def test_batch_rejects_mixed_directions(gpu):
@@ -126,3 +130,32 @@ def test_batch_rejects_mixed_directions(gpu):
                            "planck-levels_amd"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "all be syntheses or all analyses" in r.stderr, r.stderr
+
+
+def test_tuner_cache_file(gpu, tmp_path):
+    """SHARP_HIP_TUNE_CACHE: the first process tunes and appends its choice
+    (key, transform group, workgroups per CU, tile shape, flat); a second process
+    finds it there and uses the same tile shape without tuning again."""
+    import json, os, subprocess, sys, textwrap
+    cache = tmp_path / "tune.txt"
+    code = textwrap.dedent("""
+        import json, sys, torch
+        sys.path.insert(0, %r)
+        import sharp_hip as S
+        g, a = S.Geometry.healpix(16), S.AlmInfo.triangular(32)
+        al = [torch.zeros(561, dtype=torch.complex128, device="cuda") for _ in range(3)]
+        mp = [torch.zeros(3072, dtype=torch.float64, device="cuda") for _ in range(3)]
+        b = S.Batch([(S.MAP2ALM, 0, al, mp, g, a, 3)], S.DP | S.MIXED_FP32)
+        print(json.dumps(b.config()))
+        """ % os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                           "planck-levels_amd"))
+    env = dict(os.environ, SHARP_HIP_TUNE_CACHE=str(cache))
+    outs = []
+    for _ in range(2):
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=110, env=env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and len(lines[0].split()) == 5, lines
+    assert outs[0] == outs[1]

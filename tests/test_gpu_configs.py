@@ -13,9 +13,10 @@ libsharp/sharp_testsuite.c:193-229 (per component, relative L2):
   and 8192 goes through the ring FFT here);
 * the uneven batch: 24 spin-0 maps at each of Nside 128/256/512/1024 (lmax =
   2 Nside) and 8 Gauss-Legendre grids at each of lmax 255/767, as two
-  batches (synthesis, analysis) on FP32 matrix cores replayed from HIP
-  graphs: every map and round-trip a_lm within 1e-6 of the FP64 form, and
-  the FP64 form within 1e-12 of the reference on one transform per grid.
+  batches (synthesis, analysis) replayed from HIP graphs, in each form: on
+  FP32 and on BF16-split matrix cores every map and round-trip a_lm within
+  1e-6 of the FP64 form run per geometry, the FP64 batch within 1e-12 of it,
+  and the FP64 form within 1e-12 of the reference on one transform per grid.
 
 The reference runs on the host's cores (16 threads on the GPU box)."""
 import os
@@ -117,7 +118,7 @@ def test_nside2048_lmax4096_tqu8_mixed_and_fp64(gpu, ref):
         torch.cuda.empty_cache()
 
 
-def test_uneven_batch_fp32_graph_vs_fp64_and_reference(gpu, ref):
+def test_uneven_batch_vs_fp64_and_reference(gpu, ref):
     print("case: uneven batch (96 HEALPix + 16 Gauss-Legendre)", flush=True)
     specs = [("healpix", n, 2 * n, 24) for n in (128, 256, 512, 1024)]
     specs += [("gauss", 255, 255, 8), ("gauss", 767, 767, 8)]
@@ -128,20 +129,9 @@ def test_uneven_batch_fp32_graph_vs_fp64_and_reference(gpu, ref):
                 else S.Geometry.iso("gauss", lmax + 1, 2 * lmax + 1))
         ainfo = S.AlmInfo.triangular(lmax)
         alms = spin0_alms(torch, lmax, nt, gen, gpu)
-        jobs.append(dict(spec=(kind, n, lmax, nt), geom=geom, ainfo=ainfo, alms=alms,
-                         maps=_zeros_maps(nt, geom.npix, gpu),
-                         out=[torch.zeros_like(alms[0]) for _ in range(nt)]))
-    flags = S.DP | S.MIXED_FP32
-    bs = S.Batch([(S.ALM2MAP, 0, j["alms"], j["maps"], j["geom"], j["ainfo"], j["spec"][3])
-                  for j in jobs], flags, use_graph=True)
-    ba = S.Batch([(S.MAP2ALM, 0, j["out"], j["maps"], j["geom"], j["ainfo"], j["spec"][3])
-                  for j in jobs], flags, use_graph=True)
-    assert bs.config()["graph"] and ba.config()["graph"]
-    for rep in range(2):
-        bs.execute()
-        ba.execute()
-        torch.cuda.synchronize()
-    worst = 0.0
+        jobs.append(dict(spec=(kind, n, lmax, nt), geom=geom, ainfo=ainfo, alms=alms))
+    # the FP64 form per geometry, and transform 0 of every grid against the
+    # reference
     for j in jobs:
         kind, n, lmax, nt = j["spec"]
         m64 = _zeros_maps(nt, j["geom"].npix, gpu)
@@ -149,12 +139,7 @@ def test_uneven_batch_fp32_graph_vs_fp64_and_reference(gpu, ref):
         a64 = [torch.zeros_like(j["alms"][0]) for _ in range(nt)]
         S.execute_device(S.MAP2ALM, 0, a64, m64, j["geom"], j["ainfo"], nt, S.DP)
         torch.cuda.synchronize()
-        e = [_trel(x, y) for x, y in zip(j["maps"], m64)]
-        e += [_trel(x, y) for x, y in zip(j["out"], a64)]
-        print(kind, n, lmax, "mixed batch vs FP64: max %.3g" % max(e), flush=True)
-        assert 1e-12 < min(e) and max(e) <= 1e-6, (j["spec"], max(e), min(e))
-        worst = max(worst, max(e))
-        # transform 0 of every grid: the FP64 form against the reference
+        j["m64"], j["a64"] = m64, a64
         og = ref.healpix(n) if kind == "healpix" else ref.iso("gauss", lmax + 1, 2 * lmax + 1)
         oa = ref.triangular(lmax)
         rm = [np.zeros(j["geom"].npix)]
@@ -165,6 +150,30 @@ def test_uneven_batch_fp32_graph_vs_fp64_and_reference(gpu, ref):
         ref.destroy_alm(oa)
         er = (_rel(m64[0].cpu().numpy(), rm[0]), _rel(a64[0].cpu().numpy(), ra[0]))
         assert max(er) <= 1e-12, (j["spec"], er)
-    print("worst mixed error %.3g" % worst)
-    bs.close()
-    ba.close()
+    # every form as two batches (synthesis, analysis) replayed from graphs
+    for form, flag, tol in (("fp32", S.MIXED_FP32, 1e-6), ("bf16x3", S.MIXED_BF16X3, 1e-6),
+                            ("fp64", 0, 1e-12)):
+        for j in jobs:
+            j["maps"] = _zeros_maps(j["spec"][3], j["geom"].npix, gpu)
+            j["out"] = [torch.zeros_like(j["alms"][0]) for _ in range(j["spec"][3])]
+        bs = S.Batch([(S.ALM2MAP, 0, j["alms"], j["maps"], j["geom"], j["ainfo"], j["spec"][3])
+                      for j in jobs], S.DP | flag, use_graph=True)
+        ba = S.Batch([(S.MAP2ALM, 0, j["out"], j["maps"], j["geom"], j["ainfo"], j["spec"][3])
+                      for j in jobs], S.DP | flag, use_graph=True)
+        assert bs.config()["graph"] and ba.config()["graph"]
+        for rep in range(2):
+            bs.execute()
+            ba.execute()
+            torch.cuda.synchronize()
+        worst = 0.0
+        for j in jobs:
+            e = [_trel(x, y) for x, y in zip(j["maps"], j["m64"])]
+            e += [_trel(x, y) for x, y in zip(j["out"], j["a64"])]
+            assert max(e) <= tol, (form, j["spec"], max(e))
+            if form != "fp64":
+                assert min(e) > 1e-12, (form, j["spec"], "matrix-core form did not run")
+            worst = max(worst, max(e))
+        print(form, "batch vs FP64 per geometry: worst %.3g" % worst, bs.config(), ba.config(),
+              flush=True)
+        bs.close()
+        ba.close()
