@@ -441,9 +441,9 @@ def main():
                     help="also run a 1 GiB device copy (calibrates HBM counters)")
     ap.add_argument("--streams", type=int, default=1, choices=[1, 2],
                     help="streams for the T and Q/U transforms of a step")
-    ap.add_argument("--host-path", action="store_true",
-                    help="also time sharp_execute on host arrays (H2D/D2H copies "
-                         "included; reported beside value, never as value)")
+    ap.add_argument("--no-host-path", dest="host_path", action="store_false",
+                    help="skip timing sharp_execute on host arrays (H2D/D2H copies "
+                         "included; reported beside value as host_path, never as value)")
     ap.add_argument("--workload", default="tqu", choices=["tqu", "batch", "smoothing"],
                     help="tqu: the headline workload; batch: the uneven batch "
                          "(96 HEALPix maps of four sizes + 16 Gauss-Legendre grids); "

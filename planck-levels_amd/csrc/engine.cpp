@@ -58,26 +58,45 @@ namespace shp {
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
-  void *get(size_t bytes)
+  bool pooled = false;   // from hipMallocAsync
+  // A buffer of at least `bytes`.  Buffers that belong to one launch stream
+  // (ordered) grow in stream order: the old one is freed on that stream after
+  // the work already queued there, with no device-wide wait.  Others wait
+  // for the device before the old buffer is freed.
+  void *get(size_t bytes, hipStream_t st = nullptr, bool ordered = false)
   {
     if (bytes == 0) bytes = 16;
     if (bytes > cap) {
-      // the old buffer may still be in use by work queued on any stream
       if (p) {
-        HIP_CHECK(hipDeviceSynchronize());
-        HIP_CHECK(hipFree(p));
+        if (ordered) {
+          HIP_CHECK(hipFreeAsync(p, st));
+        } else {
+          HIP_CHECK(hipDeviceSynchronize());
+          HIP_CHECK(pooled ? hipFreeAsync(p, nullptr) : hipFree(p));
+          if (pooled) HIP_CHECK(hipStreamSynchronize(nullptr));
+        }
       }
       p = nullptr;
-      HIP_CHECK(hipMalloc(&p, bytes));
+      if (ordered) HIP_CHECK(hipMallocAsync(&p, bytes, st));
+      else HIP_CHECK(hipMalloc(&p, bytes));
+      pooled = ordered;
       cap = bytes;
     }
     return p;
   }
   void release()
   {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pooled) {
+        (void)hipFreeAsync(p, nullptr);
+        (void)hipStreamSynchronize(nullptr);
+      } else {
+        (void)hipFree(p);
+      }
+    }
     p = nullptr;
     cap = 0;
+    pooled = false;
   }
 };
 
@@ -1014,7 +1033,7 @@ static double *upload_lfac(Ctx &c, const Job &j, hipStream_t st)
     c.lfac_cap = bytes;
   }
   for (size_t l = 0; l < n.size(); ++l) c.lfac_host[l] = n[l] * j.lfac[l];
-  double *d = static_cast<double *>(c.bufs(st).lfac.get(bytes));
+  double *d = static_cast<double *>(c.bufs(st).lfac.get(bytes, st, true));
   HIP_CHECK(hipMemcpyAsync(d, c.lfac_host, bytes, hipMemcpyHostToDevice, st));
   HIP_CHECK(hipEventRecord(c.lfac_ev, st));
   return d;
@@ -1318,15 +1337,17 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
   GeomPlan &gp = get_geom(c, j.g);
   Ctx::StreamBufs *sb = w ? nullptr : &c.bufs(st);
   double2 *phase = w ? w->phase
-                     : static_cast<double2 *>(sb->phase.get(phase_elems(j, gp) * sizeof(double2)));
+                     : static_cast<double2 *>(
+                           sb->phase.get(phase_elems(j, gp) * sizeof(double2), st, true));
   double2 *almtmp = w ? w->almtmp
-                      : static_cast<double2 *>(sb->almtmp.get(almtmp_elems(j) * sizeof(double2)));
+                      : static_cast<double2 *>(
+                            sb->almtmp.get(almtmp_elems(j) * sizeof(double2), st, true));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
 
   const bool mixed = is_mixed(j.flags);
   if (j.type != SHARP_ALM2MAP) {
     stage_rings_to_phase(c, s, st);
-    void *part = w ? w->part : sb->part.get(part_bytes(j, (int)s.ml->hgmax.size()));
+    void *part = w ? w->part : sb->part.get(part_bytes(j, (int)s.ml->hgmax.size()), st, true);
     stage_analysis(c, s, part, st);
     return;
   }
@@ -1340,7 +1361,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
       base = w->bs_base;
     } else {
       BsPlan &bp = get_bsplan(c, j.lmax, j.mmax, j.spin, ncol);
-      bs = static_cast<unsigned *>(sb->bsplit.get((size_t)bp.total * sizeof(unsigned)));
+      bs = static_cast<unsigned *>(sb->bsplit.get((size_t)bp.total * sizeof(unsigned), st, true));
       base = bp.d;
     }
     s.aa.bsplit = bs;
@@ -1856,6 +1877,7 @@ void sharp_execute(sharp_jobtype type, int spin, void *alm, void *map,
   const bool nofft = j.flags & SHARP_NO_FFT;
   const int64_t mreal_lo = nofft ? 2 * mlo : mlo, mreal_hi = nofft ? 2 * mhi + 1 : mhi;
   const size_t an = (size_t)(ahi - alo + 1), mn = (size_t)(mreal_hi - mreal_lo + 1);
+  // (written from the copy stream too: plain allocation, not stream-ordered)
   char *da = static_cast<char *>(c.stage_alm.get(an * esz * ncomp_alm));
   char *dm = static_cast<char *>(c.stage_map.get(mn * esz * ncomp_map));
   void **halm = static_cast<void **>(alm), **hmap = static_cast<void **>(map);
@@ -2016,7 +2038,7 @@ void sharp_hip_map2alm_iter_device(int spin, void *alm, void *map,
   const int64_t n = rhi - rlo + 1;
   const int ncomp = ja.ntrans * ja.nmaps;
   const size_t esz = (flags & SHARP_DP) ? sizeof(double) : sizeof(float);
-  char *rb = static_cast<char *>(c.bufs(st).resid.get((size_t)n * esz * ncomp));
+  char *rb = static_cast<char *>(c.bufs(st).resid.get((size_t)n * esz * ncomp, st, true));
   std::vector<void *> rmap(ncomp);
   PtrTab src{}, dst{};
   for (int i = 0; i < ncomp; ++i) {
