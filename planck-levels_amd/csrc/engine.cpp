@@ -303,6 +303,13 @@ struct GeomPlan {
   std::vector<RingFftPlan> hfft[kFftKinds * kNumFftClasses];
   int nfft[kFftKinds * kNumFftClasses] = {};
   double2 *btab = nullptr;   // Bluestein chirp spectra
+  // rings too long for every class: global-memory FFTs (long_plans), with
+  // their Bluestein kernel spectra (ltab); lg_max: log2 of the largest length
+  LongRingPlan *longp = nullptr;
+  std::vector<LongRingPlan> hlong;
+  int nlong = 0, lg_max = 0;
+  double2 *ltab = nullptr;
+  int64_t npix_long = 0;
   int *dft_list = nullptr, ndft = 0;
   int *all_list = nullptr, nall = 0;
   int64_t npix_fft = 0, npix_all = 0;   // pixels in FFT rings / in all rings
@@ -312,6 +319,8 @@ struct GeomPlan {
     for (auto &m : mlims) { (void)hipFree(m->d); (void)hipFree(m->gmax); }
     for (auto *p : fft) (void)hipFree(p);
     (void)hipFree(btab);
+    (void)hipFree(longp);
+    (void)hipFree(ltab);
     (void)hipFree(dft_list);
     (void)hipFree(all_list);
   }
@@ -422,7 +431,7 @@ struct Ctx {
   // work buffers per launch stream: calls on different streams do not share
   // them and may run concurrently
   struct StreamBufs {
-    DevBuf phase, almtmp, part, resid, lfac, bsplit;
+    DevBuf phase, almtmp, part, resid, lfac, bsplit, lscratch;
   };
   std::map<hipStream_t, std::unique_ptr<StreamBufs>> sbufs;
   StreamBufs &bufs(hipStream_t st)
@@ -445,6 +454,9 @@ static std::map<int, std::unique_ptr<Ctx>> g_ctx;
 // Set (under g_mutex) while sharp_hip_execute_device_noalloc runs: a table
 // that is not cached yet is then an error instead of being built.
 static bool g_noalloc = false;
+
+// long-ring FFT slots of sharp_hip_execute_device_noalloc
+constexpr int kNoallocLongSlots = 8;
 
 static void need_build(const char *what)
 {
@@ -531,6 +543,32 @@ static int fft_plan(int rid, int n, RingFftPlan &pl)
   return -1;
 }
 
+// Plan of a ring too long for every LDS class: one FFT of length 2^lg,
+// n itself or (Bluestein) the smallest power of two >= 2n-1.  False for
+// rings longer than 2^21 pixels (the direct sums take those).
+static bool long_plan(int rid, int n, LongRingPlan &pl)
+{
+  pl = LongRingPlan{rid, n, 0, 0, -1};
+  if (n < 2) return false;
+  const bool pow2 = (n & (n - 1)) == 0;
+  const int64_t target = pow2 ? n : 2 * (int64_t)n - 1;
+  while (((int64_t)1 << pl.lg) < target) ++pl.lg;
+  pl.bluestein = pow2 ? 0 : 1;
+  return pl.lg <= 22;
+}
+
+// the LDS class of a ring of n pixels on a device with max_lds bytes of LDS
+// per workgroup (-1: none fits)
+static int ring_class(int rid, int n, int max_lds, RingFftPlan &pl)
+{
+  int cl = fft_plan(rid, n, pl);
+  // the class's FFT buffer (with its pad of one entry in 8) plus room to
+  // stage 512 phases (ring_fft.hip)
+  if (cl >= 0 && (size_t)((512 << cl) + (64 << cl) + 512) * sizeof(double2) + 256 > (size_t)max_lds)
+    cl = -1;
+  return cl;
+}
+
 // --- geometry ---------------------------------------------------------------
 static std::vector<char> geom_key(const sharp_geom_info *g)
 {
@@ -598,14 +636,15 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     if (n <= 0) continue;
     all.push_back(rid);
     RingFftPlan pl;
-    int cl = fft_plan(rid, n, pl);
-    // the class's FFT buffer (with its pad of one entry in 8) plus room to
-    // stage 512 phases (ring_fft.hip)
-    if (cl >= 0 && (size_t)((512 << cl) + (64 << cl) + 512) * sizeof(double2) + 256 > (size_t)c.max_lds)
-      cl = -1;
+    const int cl = ring_class(rid, n, c.max_lds, pl);
     gp->npix_all += n;
-    if (cl < 0) dft.push_back(rid);
-    else {
+    LongRingPlan lp;
+    if (cl < 0 && long_plan(rid, n, lp)) {
+      gp->hlong.push_back(lp);
+      gp->npix_long += n;
+    } else if (cl < 0) {
+      dft.push_back(rid);
+    } else {
       // radix set: 2 only 8, 1 only 8, 4, 2 (and 16), 0 any
       int rs = 2;
       for (int q = 0; q < pl.nrad; ++q) {
@@ -658,6 +697,44 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     launch_bluestein_table(uniq, gp->btab, c.stream);
     HIP_CHECK(hipStreamSynchronize(c.stream));
     for (auto *d : dev) (void)hipFree(d);
+  }
+  if (!gp->hlong.empty()) {
+    // Bluestein kernel spectra of the long rings, one block per length
+    std::map<int, int64_t> loffs;
+    std::vector<LongRingPlan> uniq;
+    int64_t ltot = 0;
+    for (auto &lp : gp->hlong) {
+      gp->lg_max = std::max(gp->lg_max, lp.lg);
+      if (!lp.bluestein) continue;
+      auto it = loffs.find(lp.n);
+      if (it == loffs.end()) {
+        it = loffs.emplace(lp.n, ltot).first;
+        ltot += (int64_t)1 << lp.lg;
+        LongRingPlan u = lp;
+        u.boff = it->second;
+        uniq.push_back(u);
+      }
+      lp.boff = it->second;
+    }
+    // longest first: they take longest
+    std::stable_sort(gp->hlong.begin(), gp->hlong.end(),
+                     [](const LongRingPlan &x, const LongRingPlan &y) { return x.lg > y.lg; });
+    gp->nlong = (int)gp->hlong.size();
+    gp->longp = upload(gp->hlong);
+    if (ltot > 0) {
+      HIP_CHECK(hipMalloc(&gp->ltab, (size_t)ltot * sizeof(double2)));
+      int lgu = 0;
+      for (auto &u : uniq) lgu = std::max(lgu, u.lg);
+      const int nwg = std::min<int>((int)uniq.size(), 256);
+      const int64_t slot = 2 * ((int64_t)1 << lgu);
+      double2 *tmp = nullptr;
+      HIP_CHECK(hipMalloc(&tmp, (size_t)nwg * slot * sizeof(double2)));
+      LongRingPlan *du = upload(uniq);
+      launch_long_table(du, (int)uniq.size(), tmp, slot, nwg, gp->ltab, c.stream);
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+      (void)hipFree(du);
+      (void)hipFree(tmp);
+    }
   }
   gp->ndft = (int)dft.size();
   gp->dft_list = upload(dft);
@@ -981,6 +1058,10 @@ struct JobState {
   int ncomp_alm = 0, ncomp_map = 0;
   double triples = 0., c0 = 0., c1 = 0.;
   double ring_bytes_fft = 0., ring_bytes_all = 0., ring_bytes_dft = 0., alm_bytes = 0.;
+  double ring_bytes_long = 0.;
+  // scratch of the long-ring FFTs: lslots slots of 2 * 2^lg_max values
+  double2 *lscratch = nullptr;
+  int lslots = 0;
   // the cached plans the arguments above point into, held for as long as
   // the state lives (a batch outlives sharp_hip_release)
   std::vector<std::shared_ptr<void>> keep;
@@ -1087,6 +1168,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   aa.ngroups = (int)ml.hgmax.size();
   aa.c0 = 0;
   aa.npc = ncomp_alm;
+  aa.m0 = 0;
+  aa.m1 = j.mmax + 1;
+  aa.part_row0 = 0;
+  aa.part_rows = tri_ofs(j.mmax + 1, j.lmax);
 
   RingArgs &ra = s.ra;
   ra.nslots = gp.nslots;
@@ -1133,6 +1218,10 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   la.phase = phase;
   la.ngroups = (int)ml.hgmax.size();
   la.gmax = ml.gmax;
+  la.m0 = 0;
+  la.m1 = j.mmax + 1;
+  la.part_row0 = 0;
+  la.part_rows = tri_ofs(j.mmax + 1, j.lmax);
   la.bsplit = nullptr;
   la.bs_base = nullptr;
   la.bs_ncol = 0;
@@ -1157,7 +1246,9 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
       (double)ncomp_map * ((double)nring_fft * (j.mmax + 1) * 16. + (double)gp.npix_fft * esz);
   s.ring_bytes_all =
       (double)ncomp_map * ((double)gp.nall * (j.mmax + 1) * 16. + (double)gp.npix_all * esz);
-  s.ring_bytes_dft = std::max(0., s.ring_bytes_all - s.ring_bytes_fft);
+  s.ring_bytes_long =
+      (double)ncomp_map * ((double)gp.nlong * (j.mmax + 1) * 16. + (double)gp.npix_long * esz);
+  s.ring_bytes_dft = std::max(0., s.ring_bytes_all - s.ring_bytes_fft - s.ring_bytes_long);
   s.alm_bytes = (double)tri_ofs(j.mmax + 1, j.lmax) * ncomp_alm * (16. + 2. * esz);
   return s;
 }
@@ -1173,15 +1264,38 @@ static bool force_ring_dft()
   return f;
 }
 
+// whether the rings go through the direct sums only
+static bool rings_direct(const Job &j) { return (j.flags & SHARP_NO_FFT) || force_ring_dft(); }
+
+// workgroups (and scratch slots) of the long-ring FFTs of a job
+static int long_slots(const GeomPlan &gp, const Job &j, int cap = 2048)
+{
+  if (!gp.nlong || rings_direct(j)) return 0;
+  return std::min(gp.nlong * j.ntrans * j.nmaps, cap);
+}
+static size_t long_scratch_bytes(const GeomPlan &gp, int slots)
+{
+  return (size_t)slots * 2 * ((size_t)1 << gp.lg_max) * sizeof(double2);
+}
+
+static LongRingArgs long_args(const JobState &s)
+{
+  const GeomPlan &gp = *s.gp;
+  return LongRingArgs{gp.longp, gp.nlong, gp.ltab, s.lscratch, 2 * ((int64_t)1 << gp.lg_max)};
+}
+
 static void stage_rings_to_phase(Ctx &c, JobState &s, hipStream_t st)
 {
   const GeomPlan &gp = *s.gp;
-  if ((s.j.flags & SHARP_NO_FFT) || force_ring_dft()) {
+  if (rings_direct(s.j)) {
     timed(c, st, "ring_dft_anal", 0., s.ring_bytes_all,
           [&] { launch_ring2phase_dft(s.ra, gp.all_list, gp.nall, st); });
   } else {
     timed(c, st, "ring_fft_anal", 0., s.ring_bytes_fft,
           [&] { launch_ring_fft(false, s.ra, s.fc, st); });
+    if (gp.nlong)
+      timed(c, st, "ring_long_anal", 0., s.ring_bytes_long,
+            [&] { launch_long_rings(false, s.ra, long_args(s), s.lslots, st); });
     if (gp.ndft)
       timed(c, st, "ring_dft_anal", 0., s.ring_bytes_dft,
             [&] { launch_ring2phase_dft(s.ra, gp.dft_list, gp.ndft, st); });
@@ -1191,12 +1305,15 @@ static void stage_rings_to_phase(Ctx &c, JobState &s, hipStream_t st)
 static void stage_phase_to_rings(Ctx &c, JobState &s, hipStream_t st)
 {
   const GeomPlan &gp = *s.gp;
-  if ((s.j.flags & SHARP_NO_FFT) || force_ring_dft()) {
+  if (rings_direct(s.j)) {
     timed(c, st, "ring_dft_synth", 0., s.ring_bytes_all,
           [&] { launch_phase2ring_dft(s.ra, gp.all_list, gp.nall, st); });
   } else {
     timed(c, st, "ring_fft_synth", 0., s.ring_bytes_fft,
           [&] { launch_ring_fft(true, s.ra, s.fc, st); });
+    if (gp.nlong)
+      timed(c, st, "ring_long_synth", 0., s.ring_bytes_long,
+            [&] { launch_long_rings(true, s.ra, long_args(s), s.lslots, st); });
     if (gp.ndft)
       timed(c, st, "ring_dft_synth", 0., s.ring_bytes_dft,
             [&] { launch_phase2ring_dft(s.ra, gp.dft_list, gp.ndft, st); });
@@ -1258,14 +1375,62 @@ static int anal_tmax(const Job &j)
                            : ((j.spin == 0) ? kF64MaxT0 : kF64MaxTS);
 }
 
-// bytes of the per-group partial sums of one analysis launch (ng groups of
-// ring pairs), 0 when the job needs none
+// The partial sums of one analysis launch are capped at SHARP_HIP_PART_CAP_MB
+// (default 32 GiB; 16 GiB at HEALPix Nside 2048, lmax 4096, 8 spin-2
+// transforms): larger analyses run in chunks of orders m, each chunk's
+// Legendre launch followed by its scatter.
+static size_t part_cap()
+{
+  static const size_t cap = [] {
+    const char *e = std::getenv("SHARP_HIP_PART_CAP_MB");
+    const long v = e ? std::atol(e) : 0;
+    return v > 0 ? (size_t)v << 20 : (size_t)32 << 30;
+  }();
+  return cap;
+}
+
+// bytes of the partial sums of one canonical row over ng ring groups
+static size_t part_row_bytes(const Job &j, int ng)
+{
+  const size_t esz = is_mixed(j.flags) ? sizeof(float) : sizeof(double);
+  const int npc = std::min(anal_tmax(j), j.ntrans) * j.nalm;
+  return (size_t)ng * 2 * npc * esz;
+}
+
+// boundaries m_0 = 0 < m_1 < ... = mmax+1 (multiples of 8 but the last) of
+// the analysis chunks whose partial sums fit the cap (at least 8 orders m
+// per chunk)
+static std::vector<int> part_chunks(const Job &j, int ng)
+{
+  const size_t per_row = part_row_bytes(j, ng), cap = part_cap();
+  std::vector<int> b{0};
+  int m = 0;
+  while (m <= j.mmax) {
+    int e = m;
+    size_t rows = 0;
+    while (e <= j.mmax) {
+      const int e2 = std::min(e + 8, j.mmax + 1);
+      const size_t add = (size_t)(tri_ofs(e2, j.lmax) - tri_ofs(e, j.lmax));
+      if (e > m && (rows + add) * per_row > cap) break;
+      rows += add;
+      e = e2;
+    }
+    b.push_back(e);
+    m = e;
+  }
+  return b;
+}
+
+// bytes of the per-group partial sums of the largest analysis launch (ng
+// groups of ring pairs), 0 when the job needs none
 static size_t part_bytes(const Job &j, int ng)
 {
   if (j.type == SHARP_ALM2MAP || (!is_mixed(j.flags) && fp64_anal_reduce())) return 0;
-  const size_t esz = is_mixed(j.flags) ? sizeof(float) : sizeof(double);
-  const int npc = std::min(anal_tmax(j), j.ntrans) * j.nalm;
-  return (size_t)ng * (size_t)tri_ofs(j.mmax + 1, j.lmax) * 2 * npc * esz;
+  const std::vector<int> b = part_chunks(j, ng);
+  size_t most = 0;
+  for (size_t i = 0; i + 1 < b.size(); ++i)
+    most = std::max(most, (size_t)(tri_ofs(b[i + 1], j.lmax) - tri_ofs(b[i], j.lmax)));
+  return most * part_row_bytes(j, ng);
 }
 
 // Analysis Legendre stage and a_lm scatter of one job, in transform groups:
@@ -1285,6 +1450,8 @@ static void stage_analysis(Ctx &c, JobState &s, void *part, hipStream_t st)
   const char *name = mixed ? mixed_name(j)
                            : (j.spin == 0 ? "legendre_anal_spin0" : "legendre_anal_spin");
   LegArgs &la = s.la;
+  const std::vector<int> mb = part_chunks(j, la.ngroups);
+  const double rows = (double)tri_ofs(j.mmax + 1, j.lmax);
   for (int t = 0; t < j.ntrans; t += tmax) {
     la.t0 = t;
     la.ntl = std::min(tmax, j.ntrans - t);
@@ -1292,23 +1459,38 @@ static void stage_analysis(Ctx &c, JobState &s, void *part, hipStream_t st)
     la.part = part;
     la.part_ld = 2 * npc;
     la.part_col0 = 0;
-    timed(c, st, name, s.triples * (s.c0 + s.c1 * la.ntl), 0.,
-          [&] { launch_manal(la, mixed ? mixed_contr(j.flags) : kContrF64, st); });
     s.aa.part = la.part;
     s.aa.part_f64 = mixed ? 0 : 1;
     s.aa.part_ld = la.part_ld;
     s.aa.part_c0 = t * j.nalm;
     s.aa.c0 = t * j.nalm;
     s.aa.npc = npc;
-    timed(c, st, "alm_scatter", 0., s.alm_bytes * npc / s.ncomp_alm,
-          [&] { launch_alm_scatter(s.aa, st); });
+    for (size_t k = 0; k + 1 < mb.size(); ++k) {
+      // the chunk's orders m, its rows at the start of the partial buffer
+      la.m0 = s.aa.m0 = mb[k];
+      la.m1 = s.aa.m1 = mb[k + 1];
+      la.part_row0 = s.aa.part_row0 = tri_ofs(mb[k], j.lmax);
+      la.part_rows = s.aa.part_rows = tri_ofs(mb[k + 1], j.lmax) - la.part_row0;
+      const double share = (double)la.part_rows / rows;
+      timed(c, st, name, share * s.triples * (s.c0 + s.c1 * la.ntl), 0.,
+            [&] { launch_manal(la, mixed ? mixed_contr(j.flags) : kContrF64, st); });
+      timed(c, st, "alm_scatter", 0., share * s.alm_bytes * npc / s.ncomp_alm,
+            [&] { launch_alm_scatter(s.aa, st); });
+    }
   }
+  // back to the whole range for any later use of the arguments
+  la.m0 = s.aa.m0 = 0;
+  la.m1 = s.aa.m1 = j.mmax + 1;
+  la.part_row0 = s.aa.part_row0 = 0;
+  la.part_rows = s.aa.part_rows = tri_ofs(j.mmax + 1, j.lmax);
 }
 
 // Work memory of one call supplied by the caller of run_device instead of
 // the per-stream buffers (sharp_hip_execute_device_noalloc).
 struct WorkPtrs {
   double2 *phase = nullptr, *almtmp = nullptr;
+  double2 *lscratch = nullptr;   // long-ring FFT slots (lslots of them)
+  int lslots = 0;
   void *part = nullptr;          // part_bytes
   unsigned *bsplit = nullptr;    // bs_words: the BF16 pieces of the a_lm
   int64_t *bs_base = nullptr;    // [mmax+1] word offsets (filled on the device)
@@ -1343,6 +1525,15 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
                       : static_cast<double2 *>(
                             sb->almtmp.get(almtmp_elems(j) * sizeof(double2), st, true));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
+  if (w) {
+    s.lslots = std::min(w->lslots, long_slots(gp, j));
+    s.lscratch = w->lscratch;
+  } else {
+    s.lslots = long_slots(gp, j);
+    if (s.lslots)
+      s.lscratch = static_cast<double2 *>(
+          sb->lscratch.get(long_scratch_bytes(gp, s.lslots), st, true));
+  }
 
   const bool mixed = is_mixed(j.flags);
   if (j.type != SHARP_ALM2MAP) {
@@ -1459,6 +1650,7 @@ struct BatchPlan {
   std::vector<Job> jobs;
   std::vector<std::vector<void *>> almp, mapp;
   std::vector<DevBuf> phase, almtmp, part, bsplit, gmax2;
+  DevBuf lscratch;   // long-ring FFT slots, shared by the jobs (run in turn)
   std::vector<JobState> st;
   // per-job launch arguments for each analysis ring block (4, 2 wavefronts):
   // their ring groups and group maxima of mlim differ
@@ -1486,6 +1678,7 @@ struct BatchPlan {
     if (cap) (void)hipStreamDestroy(cap);
     for (auto *v : {&phase, &almtmp, &part, &bsplit, &gmax2})
       for (auto &b : *v) b.release();
+    lscratch.release();
   }
 };
 
@@ -1964,7 +2157,23 @@ void sharp_hip_execute_device_noalloc(sharp_jobtype type, int spin, void *alm, v
   if (!base)
     SHP_FAIL("sharp_hip_execute_device_noalloc: call sharp_hip_prepare on this device first");
   Ctx &c = context();
+  // long-ring FFT slots (the tables of the geometry must be prepared)
+  g_noalloc = true;
+  const GeomPlan &gp = get_geom(c, j.g);
+  const int lslots = std::min(long_slots(gp, j), kNoallocLongSlots);
+  const size_t o_long = off;
+  off += align(long_scratch_bytes(gp, lslots));
+  if (off > kNoallocScratchBytes) {
+    char b[320];
+    std::snprintf(b, sizeof(b),
+                  "sharp_hip_execute_device_noalloc: this job needs %zu bytes of work memory, "
+                  "more than the %zu of the fixed scratch area (use sharp_hip_execute_device)",
+                  off, kNoallocScratchBytes);
+    SHP_FAIL(b);
+  }
   WorkPtrs w;
+  w.lscratch = reinterpret_cast<double2 *>(base + o_long);
+  w.lslots = lslots;
   w.phase = reinterpret_cast<double2 *>(base + o_phase);
   w.almtmp = reinterpret_cast<double2 *>(base + o_almtmp);
   w.part = base + o_part;
@@ -1973,7 +2182,6 @@ void sharp_hip_execute_device_noalloc(sharp_jobtype type, int spin, void *alm, v
   // no per-stage events either: the profiler creates them on demand
   const bool prof = c.prof.on;
   c.prof.on = false;
-  g_noalloc = true;
   run_device(c, j, static_cast<void *const *>(alm), static_cast<void *const *>(map), 0, 0,
              static_cast<hipStream_t>(stream), opcnt, &w);
   g_noalloc = false;
@@ -2130,6 +2338,16 @@ sharp_hip_batch *sharp_hip_batch_create(int njobs, const sharp_hip_job *jobs, in
       s.aa.bs_base = s.la.bs_base = bp.d;
       s.aa.bs_ncol = s.la.bs_ncol = ncol;
     }
+  }
+  // long-ring scratch: the largest any job needs, shared in turn
+  size_t lbytes = 0;
+  for (auto &s : b->st) {
+    s.lslots = long_slots(*s.gp, s.j);
+    lbytes = std::max(lbytes, long_scratch_bytes(*s.gp, s.lslots));
+  }
+  if (lbytes) {
+    double2 *ls = static_cast<double2 *>(b->lscratch.get(lbytes));
+    for (auto &s : b->st) s.lscratch = ls;
   }
   for (int wi = 0; wi < 2; ++wi) {
     std::vector<LegArgs> las;

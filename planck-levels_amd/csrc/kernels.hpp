@@ -36,6 +36,11 @@ struct LegArgs {
                         // (mixed form) or double (FP64 matrix-core form)
   int64_t part_ld;      // floats per partial row
   int part_col0;        // float column of this launch's first transform
+  // analysis launches cover the orders m0 <= m < m1 (m0 a multiple of 8);
+  // the partial buffer holds their rows: canonical row r of group g at
+  // (g * part_rows + r - part_row0) * part_ld
+  int m0, m1;
+  int64_t part_row0, part_rows;
   // BF16 spin-s synthesis: the a_lm split into BF16 pieces by the gather
   // (bs_pairs), else nullptr
   const unsigned *bsplit;
@@ -173,6 +178,30 @@ struct FftClasses {
 void launch_bluestein_table(const FftClasses &uniq, double2 *btab, hipStream_t st);
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st);
 
+// Rings too long for every LDS class: one complex FFT of the ring's length n
+// through global memory (power-of-two n), or Bluestein with L = 2^lg >= 2n-1.
+struct LongRingPlan {
+  int ring;        // slot*2 + hemi (for the Bluestein table: unused)
+  int n;           // pixels
+  int lg;          // log2 of the FFT length L
+  int bluestein;   // 1: n is not a power of two
+  int64_t boff;    // Bluestein: offset of the L spectrum entries in the table
+};
+struct LongRingArgs {
+  const LongRingPlan *plans;
+  int nplans;
+  const double2 *btab;   // Bluestein kernel spectra
+  double2 *scratch;      // nslots_wg slots of `slot` complex values
+  int64_t slot;          // 2 * the largest L
+};
+// nslots_wg workgroups, each with its scratch slot, take the (ring,
+// component) items in turn
+void launch_long_rings(bool synth, const RingArgs &a, const LongRingArgs &la, int nslots_wg,
+                       hipStream_t st);
+// fills btab at each plan's boff (plans with bluestein set, one per n)
+void launch_long_table(const LongRingPlan *plans, int nplans, double2 *scratch, int64_t slot,
+                       int nslots_wg, double2 *btab, hipStream_t st);
+
 // a_lm in the caller's layout <-> canonical almtmp.
 struct AlmArgs {
   int lmax, mmax, ncomp;
@@ -198,6 +227,9 @@ struct AlmArgs {
   int ngroups, c0, npc;
   int64_t part_ld;
   int part_c0;
+  // the scatter writes the orders m0 <= m < m1; partial rows as in LegArgs
+  int m0, m1;
+  int64_t part_row0, part_rows;
   // BF16 spin-s synthesis: where launch_alm_gather_split puts the BF16
   // pieces (bs_pairs), else nullptr
   unsigned *bsplit;

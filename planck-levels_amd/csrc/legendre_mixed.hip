@@ -912,8 +912,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  T *prow = static_cast<T *>(a.part) + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
-            (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
+  T *prow = static_cast<T *>(a.part) + (int64_t)g * a.part_rows * a.part_ld +
+            (int64_t)(tri_ofs(m, lmax) - m - a.part_row0) * a.part_ld + a.part_col0;
 
   // B: p1 = N+S and p2 = N-S of a ring pair rs, column n
   auto bpair = [&](int rs, int cb, double &p1, double &p2) {
@@ -1142,8 +1142,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   const bool active = valid && (a.mlim[s] >= m);
   const int n = lane & 15, kq = lane >> 4;
   const double *ph = reinterpret_cast<const double *>(a.phase);
-  T *prow = static_cast<T *>(a.part) + (int64_t)g * tri_ofs(a.mmax + 1, lmax) * a.part_ld +
-            (int64_t)(tri_ofs(m, lmax) - m) * a.part_ld + a.part_col0;
+  T *prow = static_cast<T *>(a.part) + (int64_t)g * a.part_rows * a.part_ld +
+            (int64_t)(tri_ofs(m, lmax) - m - a.part_row0) * a.part_ld + a.part_col0;
 
   // rows m <= l < mhi have no lambda: zero them
   {
@@ -1433,8 +1433,8 @@ __global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
   __shared__ __align__(16) T lds[kMixWaves0][kSlice];
   __shared__ double cbuf[2][64];
   int m, g;
-  if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin0_tile<T, NCB, CT, kMixWaves0>(a, m, g, lds, cbuf);
+  if (xcd_map_mblock(blockIdx.x, a.m1 - a.m0, a.ngroups, m, g))
+    manal_spin0_tile<T, NCB, CT, kMixWaves0>(a, a.m0 + m, g, lds, cbuf);
 }
 
 template <typename T, int NCB, int CT>
@@ -1443,8 +1443,8 @@ __global__ void __launch_bounds__(64 * kMixWavesS) k_manal_spin(LegArgs a)
   __shared__ __align__(16) T lds[kMixWavesS][2][kSlice];
   __shared__ double cbuf[2][96];
   int m, g;
-  if (xcd_map_mblock(blockIdx.x, a.mmax + 1, a.ngroups, m, g))
-    manal_spin_tile<T, NCB, CT, kMixWavesS>(a, m, g, lds, cbuf);
+  if (xcd_map_mblock(blockIdx.x, a.m1 - a.m0, a.ngroups, m, g))
+    manal_spin_tile<T, NCB, CT, kMixWavesS>(a, a.m0 + m, g, lds, cbuf);
 }
 
 // Batch forms (for_flat_item / for_group_items): synthesis items are
@@ -1634,7 +1634,8 @@ void launch_batch(const BatchLaunch &L, const LegArgs *jobs, hipStream_t st)
 void launch_manal(const LegArgs &a, int contr, hipStream_t st)
 {
   const bool f64 = contr == kContrF64, bf = contr == kContrBf16x3;
-  const dim3 grid(xcd_grid_mblock(a.mmax + 1, a.ngroups));
+  if (a.m0 % 8 || a.m1 <= a.m0 || a.m1 > a.mmax + 1) SHP_FAIL("bad m range of an analysis launch");
+  const dim3 grid(xcd_grid_mblock(a.m1 - a.m0, a.ngroups));
   const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
   if (a.ntl < 1 || cols > (f64 ? 16 : 32)) SHP_FAIL("bad transform group");

@@ -392,15 +392,16 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
 {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int npc = a.npc;
-  if (e >= tri_ofs(a.mmax + 1, a.lmax) * npc) return;
-  const int64_t row = e / npc;
-  const int cl = (int)(e - row * npc), comp = a.c0 + cl;
+  const int64_t r0 = tri_ofs(a.m0, a.lmax);
+  if (e >= (tri_ofs(a.m1, a.lmax) - r0) * npc) return;
+  const int64_t q = e / npc, row = r0 + q;
+  const int cl = (int)(e - q * npc), comp = a.c0 + cl;
   const int m = row_m(row, a.lmax);
   const int l = m + (int)(row - tri_ofs(m, a.lmax));
   double2 x;
   if (a.part) {
-    const int64_t gstride = (int64_t)tri_ofs(a.mmax + 1, a.lmax) * a.part_ld;
-    const int64_t o = row * a.part_ld + 2 * (comp - a.part_c0);
+    const int64_t gstride = a.part_rows * a.part_ld;
+    const int64_t o = (row - a.part_row0) * a.part_ld + 2 * (comp - a.part_c0);
     double re = 0., im = 0.;
     for (int g = 0; g < a.ngroups; ++g) {
       if (a.gmax[g] < m) continue;
@@ -470,7 +471,8 @@ void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_
 
 void launch_alm_scatter(const AlmArgs &a, hipStream_t s)
 {
-  const int64_t tot = tri_ofs(a.mmax + 1, a.lmax) * a.npc;
+  if (a.m0 < 0 || a.m1 <= a.m0 || a.m1 > a.mmax + 1) SHP_FAIL("bad m range of the a_lm scatter");
+  const int64_t tot = (tri_ofs(a.m1, a.lmax) - tri_ofs(a.m0, a.lmax)) * a.npc;
   const dim3 grid((unsigned)((tot + 255) / 256));
   if (a.flags & SHARP_DP) hipLaunchKernelGGL(k_alm_scatter<double>, grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_alm_scatter<float>, grid, dim3(256), 0, s, a);

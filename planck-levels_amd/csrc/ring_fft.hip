@@ -944,6 +944,240 @@ static void launch_kind(int c, bool synth, bool blue, const RingArgs &a,
   }
 }
 
+// ===========================================================================
+// Rings too long for every LDS class (n > 8192 complex values, or a Bluestein
+// length above 8192): the same sums as the direct-sum kernels,
+//   synthesis  f_j = Re sum_k S_k e^{+2 pi i jk/n},  S_k = sum_{m = k mod n} c_m,
+//   analysis   phase_m = w e^{-i m phi0} F_{m mod n},  F_k = sum_j f_j e^{-2 pi i jk/n},
+// with one complex FFT of the ring's full length n: radix-16/8/4/2 Stockham
+// passes through global memory when n is a power of two, otherwise
+// Bluestein's chirp-z form with a power-of-two convolution length L >= 2n-1
+// (jk = (j^2 + k^2 - (k-j)^2) / 2):
+//   F_k = c_k sum_j (f_j c_j) b_{k-j},  c_j = e^{-i pi j^2/n},  b_d = e^{+i pi d^2/n},
+// the convolution as FFT^+(FFT^-(a) . B) with B = FFT^-(b) / L precomputed per
+// ring length (b is even, so the synthesis, whose chirps are the conjugates,
+// uses conj(B)).  One workgroup takes one (ring, component) at a time and
+// keeps its data in its own slot of global scratch (two buffers of the
+// largest L); passes are separated by a device-scope fence and a barrier.
+// ===========================================================================
+namespace {
+
+constexpr int kLongThreads = 512;
+
+__device__ inline void long_sync()
+{
+  __threadfence();
+  __syncthreads();
+}
+
+// c_j = e^{-i pi j^2 / n}, the angle reduced exactly (j^2 mod 2n)
+__device__ inline double2 long_chirp(int64_t j, int n)
+{
+  const int64_t q = (j * j) % (2 * (int64_t)n);
+  return expipi(-(double)q / (double)n);
+}
+
+// One Stockham pass of radix R over a length-L sequence, src -> dst; p is the
+// product of the radices of the earlier passes.  Butterfly i takes
+// src[i + r L/R], multiplies entry r by e^{sign 2 pi i r k/(pR)} (k = i mod
+// p), transforms, and writes output q to dst[(i-k) R + k + q p].
+template <int R>
+__device__ inline void long_pass(const double2 *src, double2 *dst, int L, int p, int sign)
+{
+  const int T = L / R;
+  for (int i = threadIdx.x; i < T; i += kLongThreads) {
+    const int k = i & (p - 1);
+    double2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = src[i + r * T];
+    if (p > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r)
+        u[r] = cmul(u[r], expipi(sign * 2.0 * (double)(r * k) / (double)(p * R)));
+    }
+    Dft<R>::run(u, sign);
+    const int j = (i - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[j + r * p] = u[r];
+  }
+}
+
+// FFT of length L = 2^lg of x (sum_j x_j e^{sign 2 pi i jk/L}); y is scratch
+// of L entries; returns the buffer that holds the result
+__device__ inline double2 *long_fft(double2 *x, double2 *y, int lg, int sign)
+{
+  const int L = 1 << lg;
+  double2 *src = x, *dst = y;
+  int p = 1, rem = lg;
+  while (rem > 0) {
+    const int b = rem >= 4 ? 4 : rem;
+    if (b == 4) long_pass<16>(src, dst, L, p, sign);
+    else if (b == 3) long_pass<8>(src, dst, L, p, sign);
+    else if (b == 2) long_pass<4>(src, dst, L, p, sign);
+    else long_pass<2>(src, dst, L, p, sign);
+    long_sync();
+    double2 *t = src;
+    src = dst;
+    dst = t;
+    p <<= b;
+    rem -= b;
+  }
+  return src;
+}
+
+// the Bluestein convolution of the chirped sequence in x (L entries):
+// FFT^+(FFT^-(x) . B), B or conj(B); returns the buffer with the result
+__device__ inline double2 *long_convolve(double2 *x, double2 *y, int lg, const double2 *B,
+                                         bool conjB)
+{
+  const int L = 1 << lg;
+  double2 *f = long_fft(x, y, lg, -1);
+  for (int k = threadIdx.x; k < L; k += kLongThreads) {
+    const double2 b = B[k];
+    f[k] = cmul(f[k], conjB ? conj(b) : b);
+  }
+  long_sync();
+  return long_fft(f, f == x ? y : x, lg, +1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLongThreads) k_long_synth(RingArgs a, LongRingArgs la)
+{
+  double2 *X = la.scratch + (int64_t)blockIdx.x * la.slot;
+  const int nitems = la.nplans * a.ncomp;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const LongRingPlan pl = la.plans[it / a.ncomp];
+    const int comp = it % a.ncomp;
+    const int slot = pl.ring >> 1, hemi = pl.ring & 1;
+    const RingDev r = a.ring[pl.ring];
+    const int n = pl.n, L = 1 << pl.lg;
+    double2 *Y = X + L;
+    double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+    if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrtHalf;
+    const bool norot = fabs(r.phi0) < 1e-14;
+    // S_k: the weighted, rotated phases folded onto bin k = m mod n (factor 2
+    // for m > 0, the real part only for m = 0), then the conjugate chirp
+    for (int k = threadIdx.x; k < L; k += kLongThreads) {
+      double2 v = make_double2(0., 0.);
+      if (k < n) {
+        for (int m = k; m <= a.mmax; m += n) {
+          double2 c = cscale(a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)], wgt);
+          if (m == 0) {
+            c.y = 0.;
+          } else {
+            if (!norot) c = cmul(c, expi_small(m * r.phi0));
+            c = cscale(c, 2.);
+          }
+          v = cadd(v, c);
+        }
+        if (pl.bluestein) v = cmul(v, conj(long_chirp(k, n)));
+      }
+      X[k] = v;
+    }
+    long_sync();
+    const double2 *res = pl.bluestein ? long_convolve(X, Y, pl.lg, la.btab + pl.boff, true)
+                                      : long_fft(X, Y, pl.lg, +1);
+    void *mp = a.maps.p[comp];
+    for (int j = threadIdx.x; j < n; j += kLongThreads) {
+      double2 v = res[j];
+      if (pl.bluestein) v = cmul(v, conj(long_chirp(j, n)));
+      const int64_t idx = r.ofs + (int64_t)j * r.stride - a.map_base;
+      const double old = (a.flags & SHARP_ADD) ? ld<T>(mp, idx) : 0.;
+      st<T>(mp, idx, old + v.x);
+    }
+    long_sync();   // the slot is reused by the next item
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLongThreads) k_long_anal(RingArgs a, LongRingArgs la)
+{
+  double2 *X = la.scratch + (int64_t)blockIdx.x * la.slot;
+  const int nitems = la.nplans * a.ncomp;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const LongRingPlan pl = la.plans[it / a.ncomp];
+    const int comp = it % a.ncomp;
+    const int slot = pl.ring >> 1, hemi = pl.ring & 1;
+    const RingDev r = a.ring[pl.ring];
+    const int n = pl.n, L = 1 << pl.lg;
+    double2 *Y = X + L;
+    double wgt = (a.flags & SHARP_USE_WEIGHTS) ? r.weight : 1.;
+    if (a.flags & SHARP_REAL_HARMONICS) wgt *= kSqrt2;
+    const void *mp = a.maps.p[comp];
+    for (int j = threadIdx.x; j < L; j += kLongThreads) {
+      double2 v = make_double2(0., 0.);
+      if (j < n) {
+        v.x = ld<T>(mp, r.ofs + (int64_t)j * r.stride - a.map_base);
+        if (pl.bluestein) v = cscale(long_chirp(j, n), v.x);
+      }
+      X[j] = v;
+    }
+    long_sync();
+    const double2 *res = pl.bluestein ? long_convolve(X, Y, pl.lg, la.btab + pl.boff, false)
+                                      : long_fft(X, Y, pl.lg, -1);
+    const bool norot = fabs(r.phi0) < 1e-14;
+    for (int m = threadIdx.x; m <= a.mmax; m += kLongThreads) {
+      const int k = m % n;
+      double2 f = res[k];
+      if (pl.bluestein) f = cmul(f, long_chirp(k, n));
+      f = cscale(f, wgt);
+      if (!norot) f = cmul(f, expi_small(-m * r.phi0));
+      a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] = f;
+    }
+    long_sync();
+  }
+}
+
+// B = FFT^-(b) / L for each plan (one per Bluestein length n):
+// b_d = e^{+i pi d^2/n} at d and L-d for 0 <= d < n, zero elsewhere
+__global__ void __launch_bounds__(kLongThreads) k_long_table(const LongRingPlan *plans, int nplans,
+                                                             double2 *scratch, int64_t slot,
+                                                             double2 *btab)
+{
+  double2 *X = scratch + (int64_t)blockIdx.x * slot;
+  for (int it = blockIdx.x; it < nplans; it += gridDim.x) {
+    const LongRingPlan pl = plans[it];
+    const int n = pl.n, L = 1 << pl.lg;
+    for (int d = threadIdx.x; d < L; d += kLongThreads) {
+      double2 v = make_double2(0., 0.);
+      if (d < n) v = conj(long_chirp(d, n));
+      else if (d > L - n) v = conj(long_chirp(L - d, n));
+      X[d] = v;
+    }
+    long_sync();
+    const double2 *f = long_fft(X, X + L, pl.lg, -1);
+    const double s = 1.0 / (double)L;
+    for (int k = threadIdx.x; k < L; k += kLongThreads) btab[pl.boff + k] = cscale(f[k], s);
+    long_sync();
+  }
+}
+
+} // namespace
+
+void launch_long_rings(bool synth, const RingArgs &a, const LongRingArgs &la, int nslots_wg,
+                       hipStream_t st)
+{
+  if (la.nplans <= 0) return;
+  const dim3 grid(nslots_wg), block(kLongThreads);
+  if (synth) {
+    if (a.dp) hipLaunchKernelGGL(k_long_synth<double>, grid, block, 0, st, a, la);
+    else hipLaunchKernelGGL(k_long_synth<float>, grid, block, 0, st, a, la);
+  } else {
+    if (a.dp) hipLaunchKernelGGL(k_long_anal<double>, grid, block, 0, st, a, la);
+    else hipLaunchKernelGGL(k_long_anal<float>, grid, block, 0, st, a, la);
+  }
+  check_launch(synth ? "long-ring synthesis" : "long-ring analysis");
+}
+
+void launch_long_table(const LongRingPlan *plans, int nplans, double2 *scratch, int64_t slot,
+                       int nslots_wg, double2 *btab, hipStream_t st)
+{
+  if (nplans <= 0) return;
+  hipLaunchKernelGGL(k_long_table, dim3(nslots_wg), dim3(kLongThreads), 0, st, plans, nplans,
+                     scratch, slot, btab);
+  check_launch("long-ring Bluestein table");
+}
+
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st)
 {
   // largest rings first: they take longest

@@ -137,3 +137,44 @@ def test_whole_groups(product, oracle, gpu, case, form):
         for jt in (O.ALM2MAP, O.MAP2ALM):
             compare_mixed(product, oracle, healpix(32), _packed96, jt, 2, 2,
                           flags=O.DP | O.ADD, form=form)
+
+
+def test_analysis_in_m_chunks(gpu, tmp_path):
+    """SHARP_HIP_PART_CAP_MB caps the per-group partial sums of an analysis
+    launch; above it the analysis runs in chunks of orders m (Legendre launch
+    and scatter per chunk).  With a 1 MiB cap (about 20 chunks here) the
+    outputs equal those of one chunk, for the BF16-split, FP32 and FP64
+    forms."""
+    import os, subprocess, sys, textwrap
+    code = textwrap.dedent("""
+        import sys, numpy as np, torch
+        sys.path.insert(0, %r)
+        import sharp_hip as S
+        g, a = S.Geometry.healpix(256), S.AlmInfo.triangular(512)
+        rng = np.random.default_rng(4)
+        maps = [torch.tensor(rng.normal(size=g.npix), device="cuda") for _ in range(8)]
+        out = []
+        for form in (S.MIXED_BF16X3, S.MIXED_FP32, 0):
+            for spin, nc in ((0, 4), (2, 8)):
+                al = [torch.zeros(S.nalm_triangular(512), dtype=torch.complex128,
+                                  device="cuda") for _ in range(nc)]
+                S.execute_device(S.MAP2ALM, spin, al, maps[:nc], g, a, nc // (1 if spin == 0 else 2),
+                                 S.DP | form)
+                torch.cuda.synchronize()
+                out += [x.cpu().numpy() for x in al]
+        np.save(sys.argv[1], np.stack(out))
+        """ % os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                           "planck-levels_amd"))
+    res = []
+    for cap in (None, "1"):
+        env = dict(os.environ)
+        env.pop("SHARP_HIP_PART_CAP_MB", None)
+        if cap:
+            env["SHARP_HIP_PART_CAP_MB"] = cap
+        f = str(tmp_path / ("out%s.npy" % (cap or "")))
+        r = subprocess.run([sys.executable, "-c", code, f], capture_output=True, text=True,
+                           timeout=110, env=env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        res.append(np.load(f))
+    # the same partial sums added in the same order: identical
+    assert np.array_equal(res[0], res[1])

@@ -150,6 +150,33 @@ def test_ring_fft_classes(product, oracle, gpu, spin, jobtype):
     compare(product, oracle, _fft_class_geom, tri(2100), jobtype, spin, 1, O.DP)
 
 
+def _long_ring_geom(lib):
+    """Rings longer than the largest LDS FFT class (8192 complex values):
+    nph 16388 (N = 8194 = 2*17*241, Bluestein length 16384), 32768 (N =
+    16384) and the odd 16385; they take the direct-sum ring kernels."""
+    nph = np.array([16388, 32768, 16385, 16388])
+    theta = np.array([0.4, 1.1, 1.9, np.pi - 0.4])
+    stride = np.ones(len(nph), np.int64)
+    ofs = np.concatenate([[0], np.cumsum(nph)[:-1]]).astype(np.int64)
+    phi0 = np.array([0.0, 0.3, 1.0, 0.0])
+    wgt = np.linspace(0.5, 1.5, len(nph))
+    return lib.general_geom(nph, ofs, stride, phi0, theta, wgt)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
+def test_rings_beyond_fft_classes(product, oracle, gpu, spin, jobtype):
+    # every ring longer than 2 mmax + 1: the phases are padded, not folded
+    compare(product, oracle, _long_ring_geom, tri(2100), jobtype, spin, 1, O.DP)
+
+
+@pytest.mark.parametrize("jobtype", [O.ALM2MAP, O.MAP2ALM])
+def test_rings_beyond_fft_classes_folded(product, oracle, gpu, jobtype):
+    # mmax 8300: the rings of 16388 and 16385 pixels are shorter than
+    # 2 mmax + 1, so their phases fold onto the ring's bins
+    compare(product, oracle, _long_ring_geom, tri(8300), jobtype, 0, 1, O.DP)
+
+
 def test_packed(product, oracle, gpu):
     def packed(lib):
         lmax = 12
