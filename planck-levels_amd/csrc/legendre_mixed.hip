@@ -94,34 +94,77 @@ __device__ inline void row_transpose4(float v[4])
 // at most 8 significant bits, so l is exact).  Eight values per operand of
 // v_mfma_f32_16x16x32_bf16.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 struct Bf3 { bf16x8 h, m, l; };
-__device__ inline Bf3 split3(const float (&v)[8])
+
+// The BF16 pairs (-1, 0) and (0, -1) of the remainder step below, made by
+// scalar moves the compiler cannot see through: written as literals, the
+// pair (-1, 0) is encoded as the inline constant -1.0 of v_dot2c_f32_bf16,
+// which the gfx950 hardware applies as (0, -1).
+struct SplitK { bf16x2 n0, n1; };
+__device__ inline SplitK split_consts()
 {
-  u32x4 H, M, L;
+  unsigned c0, c1;
+  asm("s_mov_b32 %0, 0xbf80" : "=s"(c0));
+  asm("s_mov_b32 %0, 0xbf800000" : "=s"(c1));
+  return SplitK{__builtin_bit_cast(bf16x2, c0), __builtin_bit_cast(bf16x2, c1)};
+}
+
+// Each word packs the pieces of two values (element 2p in the low half):
+// one byte permute takes the top halves of both, and the remainder of each
+// value is one v_dot2c_f32_bf16, x + (-1) * piece, exact because the piece
+// is x's own leading bits.  3.5 VALU per value.
+__device__ inline Bf3 split3(const float (&v)[8], const SplitK &k)
+{
+  // in phases over the four words, so that each result is used four
+  // instructions after it is written (a dot2c result read by the next
+  // instruction costs wait states).  The words are plain scalars until the
+  // end: read back from a u32x4 for the dot products, every word came out
+  // as word 0 in the compiled code.
+  unsigned H[4], M[4], L[4];
+  float r[8], t[8];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    H[p] = __builtin_amdgcn_perm(__float_as_uint(v[2 * p + 1]), __float_as_uint(v[2 * p]),
+                                 0x07060302u);
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    unsigned hb[2], mb[2], lb[2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const float x = v[2 * p + e];
-      const unsigned xu = __float_as_uint(x);
-      const float r1 = x - __uint_as_float(xu & 0xffff0000u);
-      const unsigned ru = __float_as_uint(r1);
-      const float r2 = r1 - __uint_as_float(ru & 0xffff0000u);
-      hb[e] = xu;
-      mb[e] = ru;
-      lb[e] = __float_as_uint(r2);
-    }
-    // element 2p in the low half of word p, element 2p+1 in the high half:
-    // (hi & 0xffff0000) | (lo >> 16) as one byte permute (bytes 2,3 of lo,
-    // then bytes 2,3 of hi)
-    H[p] = __builtin_amdgcn_perm(hb[1], hb[0], 0x07060302u);
-    M[p] = __builtin_amdgcn_perm(mb[1], mb[0], 0x07060302u);
-    L[p] = __builtin_amdgcn_perm(lb[1], lb[0], 0x07060302u);
+    const bf16x2 hp = __builtin_bit_cast(bf16x2, H[p]);
+    r[2 * p] = __builtin_amdgcn_fdot2_f32_bf16(hp, k.n0, v[2 * p], false);
+    r[2 * p + 1] = __builtin_amdgcn_fdot2_f32_bf16(hp, k.n1, v[2 * p + 1], false);
   }
-  return Bf3{__builtin_bit_cast(bf16x8, H), __builtin_bit_cast(bf16x8, M),
-             __builtin_bit_cast(bf16x8, L)};
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    M[p] = __builtin_amdgcn_perm(__float_as_uint(r[2 * p + 1]), __float_as_uint(r[2 * p]),
+                                 0x07060302u);
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const bf16x2 mp = __builtin_bit_cast(bf16x2, M[p]);
+    t[2 * p] = __builtin_amdgcn_fdot2_f32_bf16(mp, k.n0, r[2 * p], false);
+    t[2 * p + 1] = __builtin_amdgcn_fdot2_f32_bf16(mp, k.n1, r[2 * p + 1], false);
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    L[p] = __builtin_amdgcn_perm(__float_as_uint(t[2 * p + 1]), __float_as_uint(t[2 * p]),
+                                 0x07060302u);
+  const u32x4 h = {H[0], H[1], H[2], H[3]}, mm = {M[0], M[1], M[2], M[3]},
+              l = {L[0], L[1], L[2], L[3]};
+  return Bf3{__builtin_bit_cast(bf16x8, h), __builtin_bit_cast(bf16x8, mm),
+             __builtin_bit_cast(bf16x8, l)};
+}
+
+// Scheduling pattern: NM matrix instructions, each followed by NV other VALU
+// instructions, so that a wavefront issues VALU work (the next group's
+// splits) while its own matrix products run.
+template <int NM, int NV>
+__device__ inline void sched_interleave()
+{
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+  }
 }
 // D += A.B from the six piece products of order >= 2^-16: with |m| < 2^-7|x|
 // and |l| < 2^-14|x| the three dropped ones stay below 2^-20 of each
@@ -643,6 +686,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
           // X and Y columns of this lane (the zero column for unused ones), Y
           // negated by its sign bit; rows l0+8kq.. are the 4 words from pair
           // (l0+8kq-mhi)/2 (l - mhi is even here)
+          const SplitK sk = split_consts();
           const unsigned *bsm = a.bsplit + a.bs_base[m];
           const int np = bs_pairs(lmax, st.mhi);
           const int czero = a.bs_ncol - 1, cbase = a.t0 * NALM * 2;
@@ -744,7 +788,7 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
               float v1[8], v2[8];
 #pragma unroll
               for (int p = 0; p < 8; ++p) { v1[p] = a1[8 * r + p]; v2[p] = a2[8 * r + p]; }
-              const Bf3 A1 = split3(v1), A2 = split3(v2);
+              const Bf3 A1 = split3(v1, sk), A2 = split3(v2, sk);
               // the group's sum from zero, then one addition to the total
               // (as the FP32 blocks flush every 32 l)
 #pragma unroll
@@ -902,6 +946,7 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
   typedef Mfma<T> MM;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;   // no ring pair of this group reaches m
+  const SplitK sk = split_consts();
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lmax = a.lmax, ncol = 2 * a.ntl, nslots = a.nslots;
@@ -956,8 +1001,8 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
           v1[jj] = (float)p1;
           v2[jj] = (float)p2;
         }
-        bq[0][kb][cb] = split3(v1);
-        bq[1][kb][cb] = split3(v2);
+        bq[0][kb][cb] = split3(v1, sk);
+        bq[1][kb][cb] = split3(v2, sk);
       }
   }
   // slice index of local row ll (0..31) of this lane's ring pair
@@ -1100,17 +1145,30 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
             for (int cb = 0; cb < NCB; ++cb) t[x][cb] = MM::run(A, bp[x][j][cb], t[x][cb]);
           }
       } else {
-        // A: rows x+2n (l = lc+x+2n), ring pairs 32kb+8kq..+7
+        // A: rows x+2n (l = lc+x+2n), ring pairs 32kb+8kq..+7, for the
+        // groups q = (x, kb) in turn; the LDS reads and the split of group
+        // q+1 are interleaved with the matrix products of group q
+        const float *sm = reinterpret_cast<const float *>(mine);
+        Bf3 A;
+        {
+          float av[8];
+          slice_row8(sm, 2 * n, 8 * kq, av);
+          A = split3(av, sk);
+        }
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int kb = 0; kb < 2; ++kb) {
+        for (int q = 0; q < 4; ++q) {
+          const int x = q >> 1, kb = q & 1;
+          Bf3 nA = A;
+          if (q < 3) {
             float av[8];
-            slice_row8(reinterpret_cast<const float *>(mine), x + 2 * n, 32 * kb + 8 * kq, av);
-            const Bf3 A = split3(av);
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma_bf3(A, bq[x][kb][cb], t[x][cb]);
+            slice_row8(sm, ((q + 1) >> 1) + 2 * n, 32 * ((q + 1) & 1) + 8 * kq, av);
+            nA = split3(av, sk);
           }
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma_bf3(A, bq[x][kb][cb], t[x][cb]);
+          if (q < 3) sched_interleave<6 * NCB, 28 / (6 * NCB) + 1>();
+          A = nA;
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1131,6 +1189,7 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   typedef Mfma<T> MM;
   static_assert(2 * NCB * 16 * kTS <= 64 * kLS, "tiles do not fit the slice");
   if (a.gmax[g] < m) return;
+  const SplitK sk = split_consts();
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lmax = a.lmax, ncol = 4 * a.ntl, nslots = a.nslots, sp = a.spin;
@@ -1195,8 +1254,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
           v1[jj] = (float)p1;
           v2[jj] = (float)p2;
         }
-        bq[0][kb][cb] = split3(v1);
-        bq[1][kb][cb] = split3(v2);
+        bq[0][kb][cb] = split3(v1, sk);
+        bq[1][kb][cb] = split3(v2, sk);
       }
   }
   auto sidx = [&](int ll) { return CT ? ll * kRS + lane : lane * kLS + ll; };
@@ -1245,10 +1304,23 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     if (mode == 2 && lc + 31 <= lmax) {
       // unscaled and whole: no checks; lambda of every lane stays in range
       any = true;
+      // the coefficients of two-step j+1 are read from LDS during two-step
+      // j (the unit-f2 forms need f0 and g only)
+      constexpr bool kUnit = sizeof(T) == 4;
+      auto ldq = [&](int j, Fx &f1, Fx &f2) {
+        f1.f0 = cq[6 * j];
+        f1.g = cq[6 * j + 1];
+        f1.f2 = kUnit ? 1. : cq[6 * j + 2];
+        f2.f0 = cq[6 * j + 3];
+        f2.g = cq[6 * j + 4];
+        f2.f2 = kUnit ? 1. : cq[6 * j + 5];
+      };
+      Fx q1, q2;
+      ldq(0, q1, q2);
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const Fx f1{cq[6 * j], cq[6 * j + 1], cq[6 * j + 2]};
-        const Fx f2{cq[6 * j + 3], cq[6 * j + 4], cq[6 * j + 5]};
+        const Fx f1 = q1, f2 = q2;
+        if (j < 15) ldq(j + 1, q1, q2);
         const double rp0 = rec2p, rm0 = rec2m;
         double rp1, rm1;
         if constexpr (sizeof(T) == 4) {
@@ -1363,20 +1435,39 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
             }
           }
       } else {
+        // groups q = (x, kb) in turn; the LDS reads and the splits of group
+        // q+1 are interleaved with the matrix products of group q
+        const float *sw = reinterpret_cast<const float *>(mw);
+        const float *sx = reinterpret_cast<const float *>(mx);
+        Bf3 Aw, Ax;
+        {
+          float aw[8], ax[8];
+          slice_row8(sw, 2 * n, 8 * kq, aw);
+          slice_row8(sx, 2 * n, 8 * kq, ax);
+          Aw = split3(aw, sk);
+          Ax = split3(ax, sk);
+        }
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int kb = 0; kb < 2; ++kb) {
+        for (int q = 0; q < 4; ++q) {
+          const int x = q >> 1, kb = q & 1;
+          Bf3 nw = Aw, nx = Ax;
+          if (q < 3) {
+            const int x1 = (q + 1) >> 1, kb1 = (q + 1) & 1;
             float aw[8], ax[8];
-            slice_row8(reinterpret_cast<const float *>(mw), x + 2 * n, 32 * kb + 8 * kq, aw);
-            slice_row8(reinterpret_cast<const float *>(mx), x + 2 * n, 32 * kb + 8 * kq, ax);
-            const Bf3 Aw = split3(aw), Ax = split3(ax);
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-              tw[x][cb] = mfma_bf3(Aw, bq[x][kb][cb], tw[x][cb]);
-              tx[x][cb] = mfma_bf3(Ax, bq[x ^ 1][kb][cb], tx[x][cb]);
-            }
+            slice_row8(sw, x1 + 2 * n, 32 * kb1 + 8 * kq, aw);
+            slice_row8(sx, x1 + 2 * n, 32 * kb1 + 8 * kq, ax);
+            nw = split3(aw, sk);
+            nx = split3(ax, sk);
           }
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            tw[x][cb] = mfma_bf3(Aw, bq[x][kb][cb], tw[x][cb]);
+            tx[x][cb] = mfma_bf3(Ax, bq[x ^ 1][kb][cb], tx[x][cb]);
+          }
+          if (q < 3) sched_interleave<12 * NCB, 56 / (12 * NCB) + 1>();
+          Aw = nw;
+          Ax = nx;
+        }
       }
 #pragma unroll
       for (int x = 0; x < 2; ++x)
