@@ -1359,6 +1359,15 @@ static const char *mixed_name(const Job &j)
   return bf ? "bf16x3_anal_spin" : "mixed_anal_spin";
 }
 
+// SHARP_HIP_FP64_SYNTHESIS=valu selects the FP64 synthesis kernels that
+// accumulate on the vector ALUs (legendre.hip, up to 4 spin-0 / 2 spin-s
+// transforms per launch) instead of the FP64 matrix-core kernels
+static bool fp64_synth_valu()
+{
+  const char *e = std::getenv("SHARP_HIP_FP64_SYNTHESIS");
+  return e && std::strcmp(e, "valu") == 0;
+}
+
 // SHARP_HIP_FP64_ANALYSIS=reduce selects the FP64 analysis kernels that sum
 // over ring pairs with cross-lane reductions (legendre.hip) instead of the
 // FP64 matrix-core kernels; both are exact to FP64 rounding
@@ -1566,7 +1575,20 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
     timed(c, st, "alm_gather", 0., s.alm_bytes, [&] { launch_alm_gather_split(s.aa, st); });
   else
     stage_gather(c, s, st);
-  if (!mixed) {
+  if (!mixed && !fp64_synth_valu()) {
+    // FP64 matrix-core syntheses: up to 16 (spin 0) or 8 (spin s) transforms
+    // per launch, the recurrence once for all of them
+    const int tmax = (j.spin == 0) ? kF64SynT0 : kF64SynTS;
+    const char *name = j.deriv ? "legendre_synth_deriv1"
+                               : (j.spin == 0 ? "legendre_synth_spin0" : "legendre_synth_spin");
+    LegArgs &la = s.la;
+    for (int t = 0; t < j.ntrans; t += tmax) {
+      la.t0 = t;
+      la.ntl = std::min(tmax, j.ntrans - t);
+      timed(c, st, name, s.triples * (s.c0 + s.c1 * la.ntl), 0.,
+            [&] { launch_synth64(la, st); });
+    }
+  } else if (!mixed) {
     stage_legendre_fp64(c, s, st);
   } else {
     // mixed precision: transforms in groups of up to 16 (spin 0) or 8 (spin

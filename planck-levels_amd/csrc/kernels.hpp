@@ -98,6 +98,11 @@ void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArg
 // ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st);
 void launch_anal(const LegArgs &a, int ntg, hipStream_t st);
+// FP64 synthesis with the contraction on the FP64 matrix cores: a.ntl
+// transforms (up to kF64SynT0 for spin 0, kF64SynTS for spin s) per launch
+constexpr int kF64SynT0 = 16;
+constexpr int kF64SynTS = 8;
+void launch_synth64(const LegArgs &a, hipStream_t st);
 
 // Mixed-precision forms: FP64 recurrence, FP32 MFMA contraction over a batch
 // of up to kMixMaxT transforms per launch (a.ntl).

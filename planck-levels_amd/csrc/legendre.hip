@@ -374,6 +374,412 @@ __device__ void synth_spin_tile(const LegArgs &a, int m, int sb)
 }
 
 // ===========================================================================
+// synthesis on the FP64 matrix cores
+// ===========================================================================
+// The wavefront layout of the mixed-precision syntheses: lane i of lane row r
+// runs the recurrence of ring pair 16r+i over l in registers; each block of
+// l values is turned (row_transpose4d) into the A operands of
+// v_mfma_f64_16x16x4f64 (16 ring pairs x 4 l, one per lane row r), the a_lm
+// rows of those l are its B operand (4 l x 16 columns), and the products
+// accumulate in FP64 over all l.  The matrix cores take the contraction
+// while the VALU runs the recurrence, for up to 32 columns per launch (16
+// spin-0 or 8 spin-s transforms), so the recurrence runs once for all of
+// them instead of once per group of 2 or 4.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma64(double a, double b, d4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// On entry v[k] of lane 16r+i belongs to lane row r; on exit v[r] of lane
+// 16k+i holds what was v[k] of lane 16r+i (the 4x4 transpose of lane rows
+// against registers, on both halves of each double).
+__device__ inline void row_transpose4d(double v[4])
+{
+  unsigned lo[4], hi[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v[k]);
+    lo[k] = (unsigned)u;
+    hi[k] = (unsigned)(u >> 32);
+  }
+  auto tr = [](unsigned (&w)[4]) {
+    const auto a = __builtin_amdgcn_permlane32_swap(w[0], w[2], false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(w[1], w[3], false, false);
+    const auto c = __builtin_amdgcn_permlane16_swap(a[0], b[0], false, false);
+    const auto d = __builtin_amdgcn_permlane16_swap(a[1], b[1], false, false);
+    w[0] = c[0];
+    w[1] = c[1];
+    w[2] = d[0];
+    w[3] = d[1];
+  };
+  tr(lo);
+  tr(hi);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    v[k] = __builtin_bit_cast(double, ((unsigned long long)hi[k] << 32) | lo[k]);
+}
+
+// stores N = t1+t2 and S = sg*(t1-t2) of the 64 ring pairs of a wavefront:
+// lane n+16q holds rows 16r+4v+q (v = 0..3) of column n of accumulator r
+template <int NCB>
+__device__ inline void store_phases64(const LegArgs &a, int m, int sb, int comp0, int ncol,
+                                      double sg, const d4 (&t1)[NCB][4],
+                                      const d4 (&t2)[NCB][4])
+{
+  const int lane = threadIdx.x, n = lane & 15, q = lane >> 4;
+  double *ph = reinterpret_cast<double *>(a.phase);
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb) {
+    const int col = cb * 16 + n;
+    if (col >= ncol) continue;
+    const int comp = comp0 + (col >> 1), part = col & 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ring = sb * 64 + 16 * r + 4 * v + q;
+        if (ring >= a.nslots) continue;
+        const double x1 = t1[cb][r][v], x2 = t2[cb][r][v];
+        ph[2 * phase_index(comp, m, ring, 0, a.nslots, a.nmb) + part] = x1 + x2;
+        if (a.ispair[ring])
+          ph[2 * phase_index(comp, m, ring, 1, a.nslots, a.nmb) + part] = sg * (x1 - x2);
+      }
+  }
+}
+
+// Spin 0, blocks of 8 l: accumulator 1 takes the even rows l0+2k with the
+// a_lm rows l0+2k, accumulator 2 the odd ones (k = 0..3 the lane row); the
+// products of a block are issued during the recurrence of the next one.
+template <int NCB, bool CROW = false>
+__device__ void synth64_spin0_tile(const LegArgs &a, int m, int sb)
+{
+  const int lane = threadIdx.x;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax, nc = a.ncomp_alm, ncol = 2 * a.ntl;
+  const int slot = sb * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+  const int n = lane & 15, kq = lane >> 4;
+
+  d4 acc1[NCB][4], acc2[NCB][4];
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc1[cb][r] = acc2[cb][r] = d4{0., 0., 0., 0.};
+
+  if (wave_any(active)) {
+    typedef std::conditional_t<CROW, URow2, const double2 *> Row2;
+    const Row2 rf = Row2(reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m));
+    const int lpad = lmax + kTabPad;   // last readable entry of the row
+    // canonical a_lm row l of this m, first column of this launch; rows past
+    // lmax (up to lmax+15) are the next m's or the slack rows, masked
+    const double *arow = reinterpret_cast<const double *>(a.alm) +
+                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0) * 2;
+    bool colok[NCB];
+    int boff1[NCB], boff2[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n;
+      colok[cb] = col < ncol;
+      boff1[cb] = 2 * kq * 2 * nc + (colok[cb] ? col : 0);
+      boff2[cb] = boff1[cb] + 2 * nc;
+    }
+    auto ldB = [&](int l0, double (&b1)[NCB], double (&b2)[NCB]) {
+      const double *row = arow + (int64_t)l0 * 2 * nc;
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        b1[cb] = row[boff1[cb]];
+        b2[cb] = row[boff2[cb]];
+      }
+    };
+
+    double lam1 = 0., lam2;
+    int scale;
+    mypow(sth, m, lam2, scale);
+    const double mf = a.mfac[m];
+    lam2 *= (m & 1) ? -mf : mf;
+    normalize(lam2, scale, kFTol);
+    // a ring pair beyond its m limit contributes nothing: zero lambda, in
+    // range, so it neither contributes nor holds back the mode switches
+    if (!active) { lam2 = 0.; scale = 0; }
+
+    int l = m;
+    bool dead = false;
+    // below the IEEE range on every lane: advance without accumulating
+    while (wave_all(scale < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      const double2 r0 = rf[l], r1 = rf[l + 1];
+      lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+      lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+      if (fabs(lam2) > kFTol) { lam1 *= kFSmall; lam2 *= kFSmall; ++scale; }
+      l += 2;
+    }
+    if (!dead) {
+      double cf = corfac(scale);
+      bool scaled = wave_any(scale < 0);
+      if (!scaled) { lam1 *= cf; lam2 *= cf; cf = 1.; }
+      double2 rc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) rc[k] = rf[min(l + k, lpad)];
+      double bc1[NCB], bc2[NCB];
+      ldB(l, bc1, bc2);
+      double pv1[4] = {0., 0., 0., 0.}, pv2[4] = {0., 0., 0., 0.};
+      double pb1[NCB], pb2[NCB];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) pb1[cb] = pb2[cb] = 0.;
+      auto mfmas = [&]() {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc1[cb][r] = mfma64(pv1[r], pb1[cb], acc1[cb][r]);
+            acc2[cb][r] = mfma64(pv2[r], pb2[cb], acc2[cb][r]);
+          }
+      };
+      while (l <= lmax) {
+        const int lb = l;
+        double bn1[NCB], bn2[NCB];
+        ldB(lb + 8, bn1, bn2);
+        double2 rn[8];
+        double v1[4], v2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double o0 = 0., o1 = 0.;
+          if (l <= lmax) {
+            const double2 r0 = rc[2 * q], r1 = rc[2 * q + 1];
+            o0 = lam2 * cf;
+            lam1 = (cth * lam2) * r0.x - lam1 * r0.y;
+            o1 = (l + 1 <= lmax) ? lam1 * cf : 0.;
+            lam2 = (cth * lam1) * r1.x - lam2 * r1.y;
+            if (scaled && fabs(lam2) > kFTol) {
+              lam1 *= kFSmall; lam2 *= kFSmall; ++scale;
+              cf = corfac(scale);
+            }
+          }
+          v1[q] = o0;
+          v2[q] = o1;
+          l += 2;
+          if (q == 0) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) rn[k] = rf[min(lb + 8 + k, lpad)];
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (q == 1) mfmas();   // the previous block's products
+        }
+        row_transpose4d(v1);
+        row_transpose4d(v2);
+        {
+          const int l1 = lb + 2 * kq, l2 = l1 + 1;
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            pb1[cb] = (colok[cb] && l1 <= lmax) ? bc1[cb] : 0.;
+            pb2[cb] = (colok[cb] && l2 <= lmax) ? bc2[cb] : 0.;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { pv1[q] = v1[q]; pv2[q] = v2[q]; }
+        if (scaled && wave_all(scale >= 0)) {
+          lam1 *= cf; lam2 *= cf; cf = 1.;
+          scaled = false;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rc[k] = rn[k];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) { bc1[cb] = bn1[cb]; bc2[cb] = bn2[cb]; }
+      }
+      mfmas();
+    }
+  }
+  store_phases64<NCB>(a, m, sb, a.t0, ncol, 1., acc1, acc2);
+}
+
+// Spin s (and ALM2MAP_DERIV1), blocks of 4 l: accumulator 1 takes the rows
+// (w_l, x_l+1, w_l+2, x_l+3) with the a_lm rows (X_l, Y_l+1, X_l+2, Y_l+3),
+// accumulator 2 the other way round, where lambda_w = lambda_+ + lambda_-,
+// lambda_x = lambda_- - lambda_+, X = (E.re, E.im, B.re, B.im) and
+// Y = (-B.im, B.re, E.im, -E.re) per Q/U column (libsharp/sharp_core_inc2.c:
+// 282-306); N = t1+t2 and S = sg*(t1-t2) as in the scalar kernel.
+template <int NCB, bool DERIV, bool CROW = false>
+__device__ void synth64_spin_tile(const LegArgs &a, int m, int sb)
+{
+  const int lane = threadIdx.x;
+  if (m > a.mmax) return;
+  const int lmax = a.lmax, nc = a.ncomp_alm, sp = a.spin, ncol = 4 * a.ntl;
+  const int slot = sb * 64 + lane;
+  const bool valid = slot < a.nslots;
+  const int s = valid ? slot : a.nslots - 1;
+  const double cth = a.cth[s], sth = a.sth[s];
+  const bool active = valid && (a.mlim[s] >= m);
+  const SpinStart st = spin_start(m, sp);
+  const int n = lane & 15, kq = lane >> 4;
+
+  d4 acc1[NCB][4], acc2[NCB][4];
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc1[cb][r] = acc2[cb][r] = d4{0., 0., 0., 0.};
+
+  if (wave_any(active) && st.mhi <= lmax) {
+    typedef std::conditional_t<CROW, URow, const double *> Row;
+    const Row fxb = Row(a.tab + 3 * (tab_ofs(m, lmax) - m));
+    const int lpad = lmax + kTabPad;
+    auto fx = [&](int l) {
+      const int k = min(l, lpad);
+      return Fx{fxb[3 * k], fxb[3 * k + 1], fxb[3 * k + 2]};
+    };
+    constexpr int NALM = DERIV ? 1 : 2;
+    const double *arow = reinterpret_cast<const double *>(a.alm) +
+                         ((int64_t)(tri_ofs(m, lmax) - m) * nc + a.t0 * NALM) * 2;
+    // column col = 4t+q is (Q.re, Q.im, U.re, U.im)[q] of transform t: its
+    // X and Y entries as offsets into the row with signs (offset 0, sign 0
+    // for unused columns)
+    int xo[NCB], yo[NCB];
+    double xs[NCB], ys[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int col = cb * 16 + n, t = col >> 2, q = col & 3;
+      const bool ok = col < ncol;
+      if (!DERIV) {
+        xo[cb] = ok ? col : 0;
+        xs[cb] = ok ? 1. : 0.;
+        yo[cb] = ok ? (col ^ 3) : 0;
+        ys[cb] = ok ? ((q == 0 || q == 3) ? -1. : 1.) : 0.;
+      } else {
+        // no curl a_lm: X = (E.re, E.im, 0, 0), Y = (0, 0, E.im, -E.re)
+        xo[cb] = ok ? 2 * t + (q & 1) : 0;
+        xs[cb] = (ok && q < 2) ? 1. : 0.;
+        yo[cb] = ok ? 2 * t + (q == 2 ? 1 : 0) : 0;
+        ys[cb] = ok ? (q == 2 ? 1. : (q == 3 ? -1. : 0.)) : 0.;
+      }
+      // lane row kq reads a_lm row l0+kq
+      xo[cb] += kq * 2 * nc;
+      yo[cb] += kq * 2 * nc;
+    }
+    auto ldB = [&](int l0, double (&bx)[NCB], double (&by)[NCB]) {
+      const double *row = arow + (int64_t)l0 * 2 * nc;
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        bx[cb] = row[xo[cb]];
+        by[cb] = row[yo[cb]];
+      }
+    };
+
+    double rec1p = 0., rec1m = 0., rec2p, rec2m;
+    int scalep, scalem;
+    spin_init(cth, sth, sp, st, a.prefac[m], a.fscale[m], rec2p, rec2m, scalep, scalem);
+    if (!active) { rec2p = rec2m = 0.; scalep = scalem = 0; }
+    int l = st.mhi;
+    bool dead = false;
+    while (wave_all(scalep < 1 && scalem < 1)) {
+      if (l + 2 > lmax) { dead = true; break; }
+      rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(l + 1));
+      rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(l + 2));
+      rescale1(rec1p, rec2p, scalep);
+      rescale1(rec1m, rec2m, scalem);
+      l += 2;
+    }
+    if (!dead) {
+      double cfp = corfac(scalep), cfm = corfac(scalem);
+      bool scaled = wave_any(scalep < 0 || scalem < 0);
+      if (!scaled) {
+        rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+        cfp = cfm = 1.;
+      }
+      Fx fc[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
+      double bcx[NCB], bcy[NCB];
+      ldB(l, bcx, bcy);
+      const bool odd = (kq & 1) != 0;
+      double pv1[4] = {0., 0., 0., 0.}, pv2[4] = {0., 0., 0., 0.};
+      double pb1[NCB], pb2[NCB];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) pb1[cb] = pb2[cb] = 0.;
+      auto mfmas = [&]() {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc1[cb][r] = mfma64(pv1[r], pb1[cb], acc1[cb][r]);
+            acc2[cb][r] = mfma64(pv2[r], pb2[cb], acc2[cb][r]);
+          }
+      };
+      while (l <= lmax) {
+        const int lb = l;
+        double bnx[NCB], bny[NCB];
+        ldB(lb + 4, bnx, bny);
+        Fx fn[4];
+        double v1[4], v2[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double lw0 = 0., lx0 = 0., lw1 = 0., lx1 = 0.;
+          if (l <= lmax) {
+            const double rp0 = rec2p * cfp, rm0 = rec2m * cfm;
+            rec_step(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
+            const double rp1 = rec1p * cfp, rm1 = rec1m * cfm;
+            rec_step(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
+            if (scaled) {
+              const bool a1 = rescale1(rec1p, rec2p, scalep);
+              const bool a2 = rescale1(rec1m, rec2m, scalem);
+              if (a1 || a2) { cfp = corfac(scalep); cfm = corfac(scalem); }
+            }
+            lw0 = rp0 + rm0;
+            lx0 = rm0 - rp0;
+            if (l + 1 <= lmax) {
+              lw1 = rp1 + rm1;
+              lx1 = rm1 - rp1;
+            }
+          }
+          v1[2 * h] = lw0;
+          v1[2 * h + 1] = lx1;
+          v2[2 * h] = lx0;
+          v2[2 * h + 1] = lw1;
+          l += 2;
+          if (h == 0) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fn[k] = fx(lb + 5 + k);
+            __builtin_amdgcn_sched_barrier(0);
+            mfmas();   // the previous block's products
+          }
+        }
+        row_transpose4d(v1);
+        row_transpose4d(v2);
+        {
+          const bool okr = lb + kq <= lmax;
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) {
+            const double bx = okr ? xs[cb] * bcx[cb] : 0.;
+            const double by = okr ? ys[cb] * bcy[cb] : 0.;
+            pb1[cb] = odd ? by : bx;
+            pb2[cb] = odd ? bx : by;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { pv1[q] = v1[q]; pv2[q] = v2[q]; }
+        if (scaled && wave_all(scalep >= 0 && scalem >= 0)) {
+          rec1p *= cfp; rec2p *= cfp; rec1m *= cfm; rec2m *= cfm;
+          cfp = cfm = 1.;
+          scaled = false;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fc[k] = fn[k];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) { bcx[cb] = bnx[cb]; bcy[cb] = bny[cb]; }
+      }
+      mfmas();
+    }
+  }
+  const double sg = ((st.mhi - m + sp) & 1) ? -1. : 1.;
+  store_phases64<NCB>(a, m, sb, 2 * a.t0, ncol, sg, acc1, acc2);
+}
+
+// ===========================================================================
 // synthesis kernels: one job (grid over its m-blocks and slot blocks, the
 // slot blocks of one m-block, which read the same coefficient and a_lm rows,
 // on one XCD), or a batch (persistent grid over a work list)
@@ -393,6 +799,22 @@ __global__ void __launch_bounds__(64 * MB) k_synth_spin(LegArgs a, int nsb)
   int mb, sb;
   if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
     synth_spin_tile<TG, DERIV>(a, mb * MB + wave_y(), sb);
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(64 * MB) k_synth64_spin0(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
+    synth64_spin0_tile<NCB>(a, mb * MB + wave_y(), sb);
+}
+
+template <int NCB, bool DERIV>
+__global__ void __launch_bounds__(64 * MB) k_synth64_spin(LegArgs a, int nsb)
+{
+  int mb, sb;
+  if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
+    synth64_spin_tile<NCB, DERIV>(a, mb * MB + wave_y(), sb);
 }
 
 // items: (m-block of blockDim.y orders m, slot block) tiles, one order m per
@@ -767,6 +1189,27 @@ void launch_synth(const LegArgs &a, int ntg, hipStream_t st)
     }
   }
   check_launch("synthesis");
+}
+
+void launch_synth64(const LegArgs &a, hipStream_t st)
+{
+  const int ncol = (a.spin == 0 ? 2 : 4) * a.ntl;
+  if (a.ntl < 1 || ncol > 32) SHP_FAIL("bad transform group");
+  const int nsb = (a.nslots + 63) / 64;
+  const dim3 grid(xcd_grid(a.nmb, nsb));
+  const dim3 block(64, MB);
+  const bool two = ncol > 16;
+  if (a.spin == 0) {
+    if (two) hipLaunchKernelGGL(k_synth64_spin0<2>, grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL(k_synth64_spin0<1>, grid, block, 0, st, a, nsb);
+  } else if (a.deriv1) {
+    if (two) hipLaunchKernelGGL((k_synth64_spin<2, true>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_synth64_spin<1, true>), grid, block, 0, st, a, nsb);
+  } else {
+    if (two) hipLaunchKernelGGL((k_synth64_spin<2, false>), grid, block, 0, st, a, nsb);
+    else hipLaunchKernelGGL((k_synth64_spin<1, false>), grid, block, 0, st, a, nsb);
+  }
+  check_launch("FP64 matrix-core synthesis");
 }
 
 template <bool FLAT>
