@@ -1730,7 +1730,8 @@ static int batch_tmax(int kind, int contr)
   const bool synth = kind <= kBatchSynthD;
   if (contr != kContrF64) return spin0 ? kMixMaxT0 : kMixMaxTS;
   if (synth) return spin0 ? 4 : 2;
-  return spin0 ? kF64MaxT0 : kF64MaxTS;
+  // the batch FP64 analysis kernels hold one block of 16 columns
+  return spin0 ? kF64MaxT0 : 4;
 }
 
 // analysis wavefronts per workgroup of a ring-block choice

@@ -120,13 +120,13 @@ constexpr int kMixWaves0 = SHP_MIX_WAVES0;
 constexpr int kMixWavesS = SHP_MIX_WAVESS;
 constexpr int mix_group_slots(int spin) { return 64 * (spin == 0 ? kMixWaves0 : kMixWavesS); }
 // contraction of the matrix-core Legendre kernels: FP32 MFMA; FP64 MFMA
-// (the FP64 form: FP64 partial sums, up to 8 spin-0 / 4 spin-s transforms
+// (the FP64 form: FP64 partial sums, up to 8 spin-0 / 8 spin-s transforms
 // per launch); BF16 MFMA on exact three-piece BF16 splits of FP32 operands
 enum { kContrF32 = 0, kContrF64 = 1, kContrBf16x3 = 2 };
 void launch_msynth(const LegArgs &a, int contr, hipStream_t st);
 void launch_manal(const LegArgs &a, int contr, hipStream_t st);
 constexpr int kF64MaxT0 = 8;
-constexpr int kF64MaxTS = 4;
+constexpr int kF64MaxTS = 8;
 
 // Arguments of the ring stage (phase <-> map).
 constexpr int kMaxComp = 200;   // ntrans <= 100, two components each

@@ -1729,10 +1729,14 @@ void launch_manal(const LegArgs &a, int contr, hipStream_t st)
   const dim3 grid(xcd_grid_mblock(a.m1 - a.m0, a.ngroups));
   const dim3 block(64 * (a.spin == 0 ? kMixWaves0 : kMixWavesS));
   const int cols = (a.spin == 0 ? 2 : 4) * a.ntl;
-  if (a.ntl < 1 || cols > (f64 ? 16 : 32)) SHP_FAIL("bad transform group");
+  // FP64: spin 0 up to 16 columns; spin s up to 32 (the kernel runs at one
+  // workgroup per CU for its LDS anyway, so the second column block costs
+  // no occupancy and saves a second pass of the recurrence)
+  if (a.ntl < 1 || cols > ((f64 && a.spin == 0) ? 16 : 32)) SHP_FAIL("bad transform group");
   const bool two = cols > 16;
   if (f64) {
     if (a.spin == 0) hipLaunchKernelGGL((k_manal_spin0<double, 1, 0>), grid, block, 0, st, a);
+    else if (two) hipLaunchKernelGGL((k_manal_spin<double, 2, 0>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((k_manal_spin<double, 1, 0>), grid, block, 0, st, a);
   } else if (!a.unit_tab) {
     SHP_FAIL("mixed analysis without the unit-coefficient table");
