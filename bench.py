@@ -238,8 +238,9 @@ def roofline(stats, mixed, nt, workload=None, bf16=False):
 
 def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
     """The drop-in sharp_execute on host (numpy) arrays: every call copies its
-    inputs to HBM and its outputs back.  One TQU round trip of the batch per
-    form, after one untimed round trip."""
+    inputs to HBM and its outputs back.  The best of three TQU round trips of
+    the batch per form (the reference's best-of-N protocol,
+    libsharp/sharp_testsuite.c:554-577), after one untimed round trip."""
     hT = [t.cpu().numpy() for t in T]
     hEB = [t.cpu().numpy() for t in EB]
     res = {}
@@ -257,14 +258,16 @@ def host_path(S, torch, T, EB, geom, ainfo, batch, npix, forms):
             S.execute(S.MAP2ALM, 0, out_t, map_t, geom, ainfo, batch, flags)
             S.execute(S.MAP2ALM, 2, out_eb, map_qu, geom, ainfo, batch, flags)
         rt()
-        t0 = time.perf_counter()
-        rt()
-        dt = time.perf_counter() - t0
+        dt = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            rt()
+            dt = min(dt, time.perf_counter() - t0)
         res[form] = {"value": batch / dt, "ms_per_step": dt * 1e3}
         log("host path %s: %.1f ms/step, %.3f round trips/s" % (form, dt * 1e3, batch / dt))
     res["unit"] = "TQU round trips/s"
     res["note"] = ("sharp_execute on host arrays: H2D of inputs and D2H of outputs "
-                   "inside every call")
+                   "inside every call; best of 3 round trips")
     return res
 
 

@@ -187,7 +187,7 @@ struct HostStage {
   static constexpr int kSlots = 4;
   static constexpr size_t kChunk = size_t(64) << 20;
   char *pinned = nullptr;
-  hipEvent_t ev[kSlots] = {}, mark = nullptr;
+  hipEvent_t ev[kSlots] = {}, mark = nullptr, done[2] = {};
   bool busy[kSlots] = {};
   hipStream_t copy = nullptr;
   int next = 0;
@@ -197,6 +197,7 @@ struct HostStage {
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pinned), kSlots * kChunk));
     for (auto &e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&mark, hipEventDisableTiming));
+    for (auto &e : done) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
   }
   void release()
@@ -206,6 +207,7 @@ struct HostStage {
     (void)hipHostFree(pinned);
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(mark);
+    for (auto &e : done) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(copy);
     pinned = nullptr;
     for (auto &b : busy) b = false;
@@ -236,7 +238,13 @@ struct HostStage {
   {
     init();
     HIP_CHECK(hipEventRecord(mark, st));
-    HIP_CHECK(hipStreamWaitEvent(copy, mark, 0));
+    download_after(segs, mark);
+  }
+  // device -> host once `ready` has completed; returns when done
+  void download_after(const std::vector<Seg> &segs, hipEvent_t ready)
+  {
+    init();
+    HIP_CHECK(hipStreamWaitEvent(copy, ready, 0));
     struct Chunk { char *dst; const char *src; size_t n; int slot; };
     std::vector<Chunk> ch;
     for (const Seg &g : segs)
@@ -2105,25 +2113,55 @@ void sharp_execute(sharp_jobtype type, int spin, void *alm, void *map,
   const bool synth = j.type == SHARP_ALM2MAP, add = (j.flags & SHARP_ADD) != 0;
   const bool up_alm = synth || add || !alm_dense(alm_info, alo, ahi);
   const bool up_map = !synth || add || !map_dense(geom_info, mlo, mhi);
-  std::vector<HostStage::Seg> in, out;
-  for (int i = 0; i < ncomp_alm; ++i) {
-    dalm[i] = da + i * an * esz;
-    char *h = static_cast<char *>(halm[i]) + alo * (int64_t)esz;
-    if (up_alm) in.push_back({dalm[i], h, an * esz});
-    if (!synth) out.push_back({h, dalm[i], an * esz});
+  for (int i = 0; i < ncomp_alm; ++i) dalm[i] = da + i * an * esz;
+  for (int i = 0; i < ncomp_map; ++i) dmap[i] = dm + i * mn * esz;
+  // the segments of transforms [t0, t1)
+  auto segs = [&](int t0, int t1, std::vector<HostStage::Seg> &in,
+                  std::vector<HostStage::Seg> &out) {
+    for (int i = t0 * j.nalm; i < t1 * j.nalm; ++i) {
+      char *h = static_cast<char *>(halm[i]) + alo * (int64_t)esz;
+      if (up_alm) in.push_back({dalm[i], h, an * esz});
+      if (!synth) out.push_back({h, dalm[i], an * esz});
+    }
+    for (int i = t0 * j.nmaps; i < t1 * j.nmaps; ++i) {
+      char *h = static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz;
+      if (up_map) in.push_back({dmap[i], h, mn * esz});
+      if (synth) out.push_back({h, dmap[i], mn * esz});
+    }
+  };
+  // Inputs go through the pinned ring (host threads fill one chunk while the
+  // DMA engine moves the previous one), then the transform, then the outputs
+  // back the same way.  A large call runs as two groups of transforms, so
+  // that the host copies and DMA of one group overlap the device work of the
+  // other: upload A, transform A | upload B, transform B | download A,
+  // download B.
+  const size_t bytes = (size_t)(up_alm || !synth ? an * ncomp_alm : 0) * esz +
+                       (size_t)(up_map || synth ? mn * ncomp_map : 0) * esz;
+  static const int groups = [] {
+    const char *e = std::getenv("SHARP_HIP_HOST_GROUPS");
+    return e ? std::max(1, std::min(2, std::atoi(e))) : 2;
+  }();
+  const int ng = (groups > 1 && j.ntrans >= 2 && bytes >= (size_t(256) << 20)) ? 2 : 1;
+  std::vector<HostStage::Seg> out_prev;
+  for (int g = 0; g < ng; ++g) {
+    const int t0 = j.ntrans * g / ng, t1 = j.ntrans * (g + 1) / ng;
+    std::vector<HostStage::Seg> in, out;
+    segs(t0, t1, in, out);
+    Job jg = j;
+    jg.ntrans = t1 - t0;
+    c.host.upload(in, st);
+    run_device(c, jg, dalm.data() + t0 * j.nalm, dmap.data() + t0 * j.nmaps, alo, mlo, st,
+               nullptr);
+    HIP_CHECK(hipEventRecord(c.host.done[g], st));
+    if (g > 0) c.host.download_after(out_prev, c.host.done[g - 1]);
+    out_prev.swap(out);
   }
-  for (int i = 0; i < ncomp_map; ++i) {
-    dmap[i] = dm + i * mn * esz;
-    char *h = static_cast<char *>(hmap[i]) + mreal_lo * (int64_t)esz;
-    if (up_map) in.push_back({dmap[i], h, mn * esz});
-    if (synth) out.push_back({h, dmap[i], mn * esz});
+  c.host.download_after(out_prev, c.host.done[ng - 1]);
+  // libsharp's operation count of the whole call
+  if (opcnt) {
+    GeomPlan &gp = get_geom(c, j.g);
+    *opcnt = op_count(j, gp, get_mlim(gp, j.lmax, j.spin));
   }
-  // inputs through the pinned ring (host threads fill one chunk while the
-  // DMA engine moves the previous one), the transform, outputs back the
-  // same way
-  c.host.upload(in, st);
-  run_device(c, j, dalm.data(), dmap.data(), alo, mlo, st, opcnt);
-  c.host.download(out, st);
   if (time)
     *time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
