@@ -480,8 +480,9 @@ __device__ void msynth_spin0_tile(const LegArgs &a, int m, int sb)
 // ===========================================================================
 // CT: 0 FP32 matrix cores; 1 BF16 matrix cores on three-piece splits in the
 // unscaled part (groups of 32 l), FP32 for the scaled blocks and the tail
+// cst (CT 1): this wavefront's 64 entries of LDS for the coefficient groups
 template <int NCB, bool DERIV, int CT, bool CROW = false>
-__device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
+__device__ void msynth_spin_tile(const LegArgs &a, int m, int sb, double2 *cst)
 {
   const int lane = threadIdx.x;
   if (m > a.mmax) return;
@@ -711,9 +712,31 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
             py[cb] = bsm + (int64_t)3 * cy * np + 4 * kq;
             sgy[cb] = neg ? 0x80008000u : 0u;
           }
+          // Recurrence coefficients (f0, g) of l0+1 .. l0+32 in this
+          // wavefront's LDS slice, staged one group ahead: lanes 0..31 load
+          // the next group's 32 entries with vector loads at the top of a
+          // group and store them at its end, so that their latency is
+          // covered by a whole group; the group reads them at wave-uniform
+          // addresses into VGPRs, the operands of the FMAs (an FP64 FMA
+          // takes one scalar operand, so scalar coefficients cost a move).
+          const double *fxr = a.tab + 3 * (tab_ofs(m, lmax) - m);
+          auto fetch = [&](int l1) {
+            const int k = min(l1 + (lane & 31), lpad);
+            return make_double2(fxr[3 * k], fxr[3 * k + 1]);
+          };
+          {
+            const double2 c = fetch(l + 1);
+            if (lane < 32) cst[lane] = c;
+          }
+          int cbi = 0;
           while (l + 31 <= lmax) {
             const int l0 = l;
             const int r2 = (l0 - st.mhi) >> 1;
+            const double2 cnext = fetch(l0 + 33);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const double2 *cq = cst + 32 * cbi;
             u32x4 PX[NCB][3], PY[NCB][3];
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb)
@@ -724,33 +747,25 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
               }
             float a1[32], a2[32];
 #pragma unroll
-            for (int g4 = 0; g4 < 8; ++g4) {
-              const int lb = l;
-              Fx fn[4];
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const double rp0 = rec2p, rm0 = rec2m;
-                rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, fc[2 * h]);
-                const double rp1 = rec1p, rm1 = rec1m;
-                rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, fc[2 * h + 1]);
-                const int k = 4 * g4 + 2 * h;
-                // combinations in FP32 of the FP32-rounded chains
-                const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
-                a1[k] = p0 + m0;
-                a2[k] = m0 - p0;
-                a1[k + 1] = m1 - p1;
-                a2[k + 1] = p1 + m1;
-                if (h == 0) {
-                  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                  for (int q = 0; q < 4; ++q) fn[q] = fxu(lb + 5 + q);
-                  __builtin_amdgcn_sched_barrier(0);
-                }
-              }
-#pragma unroll
-              for (int q = 0; q < 4; ++q) fc[q] = fn[q];
-              l += 4;
+            for (int k = 0; k < 32; k += 2) {
+              const double2 f1 = cq[k], f2 = cq[k + 1];
+              const double rp0 = rec2p, rm0 = rec2m;
+              rec_step_unit(rec1p, rec1m, rec2p, rec2m, cth, Fx{f1.x, f1.y, 1.});
+              const double rp1 = rec1p, rm1 = rec1m;
+              rec_step_unit(rec2p, rec2m, rec1p, rec1m, cth, Fx{f2.x, f2.y, 1.});
+              // combinations in FP32 of the FP32-rounded chains
+              const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
+              a1[k] = p0 + m0;
+              a2[k] = m0 - p0;
+              a1[k + 1] = m1 - p1;
+              a2[k + 1] = p1 + m1;
+              // the next group's coefficients to the other slice (last read
+              // in the previous group), a quarter of the recurrence after
+              // their load was issued: their registers are free again
+              // before the products below
+              if (k == 8 && lane < 32) cst[32 * (cbi ^ 1) + lane] = cnext;
             }
+            l += 32;
             // ring group r's values at k = 8kq..8kq+7 to lane row kq, slot r
 #pragma unroll
             for (int p = 0; p < 8; ++p) {
@@ -797,7 +812,11 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb)
                 tot2[cb][r] += mfma_bf3(A2, B2[cb], f4{0.f, 0.f, 0.f, 0.f});
               }
             }
+            cbi ^= 1;
           }
+          // the general blocks of the tail read their coefficients from fc
+#pragma unroll
+          for (int k = 0; k < 4; ++k) fc[k] = fx(l + 1 + k);
           ldB(l, bcx, bcy);
         }
       } else
@@ -1513,9 +1532,10 @@ __global__ void __launch_bounds__(64 * MB) k_msynth_spin0(LegArgs a, int nsb)
 template <int NCB, bool DERIV, int CT>
 __global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
 {
+  __shared__ double2 cst[CT ? MB : 1][64];
   int mb, sb;
   if (xcd_map(blockIdx.x, a.nmb, nsb, mb, sb))
-    msynth_spin_tile<NCB, DERIV, CT>(a, mb * MB + wave_y(), sb);
+    msynth_spin_tile<NCB, DERIV, CT>(a, mb * MB + wave_y(), sb, cst[CT ? wave_y() : 0]);
 }
 
 template <typename T, int NCB, int CT>
@@ -1565,9 +1585,10 @@ __global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *__restr
                                                           const int *__restrict__ xofs,
                                                           int *__restrict__ ctr)
 {
+  __shared__ double2 cst[CT ? MB : 1][64];
   auto tile = [&](const BatchItem &it) {
     msynth_spin_tile<NCB, DERIV, CT, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(),
-                                           it.minor);
+                                           it.minor, cst[CT ? wave_y() : 0]);
   };
   if constexpr (FLAT) {
     for_flat_item(items, xofs, tile);
