@@ -1318,7 +1318,11 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     const int lc = st.mhi + 32 * c;
     // the other buffer was last read in chunk c-1, before its barriers
     stage_coef(lc + 32, cbuf[(c + 1) & 1]);
-    const double *cq = cbuf[c & 1];
+    // the buffer index as a VGPR value (an identity DPP move, which the
+    // compiler does not take as uniform): the reads below then address the
+    // buffer as one VGPR plus immediate offsets, not as a scalar address
+    // moved into a VGPR for every read
+    const double *cq = &cbuf[0][0] + 96 * __builtin_amdgcn_mov_dpp(c & 1, 0xE4, 0xF, 0xF, false);
     bool any = false;
     if (mode == 2 && lc + 31 <= lmax) {
       // unscaled and whole: no checks; lambda of every lane stays in range
