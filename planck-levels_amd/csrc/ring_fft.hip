@@ -264,7 +264,13 @@ __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
                                      const Pre &pre = Pre(), const Post &post = Post())
 {
   constexpr int BPT = (E + R - 1) / R;
-  const int tid = threadIdx.x;
+  // the thread index through an empty asm, so that nothing of this pass
+  // (addresses, twiddles) is computed ahead of it: with compile-time
+  // lengths and strides it all depends on the thread index alone, and the
+  // compiler would hoist every pass's share out of the component loop and
+  // spill it
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   const int nb = N / R;
   double2 v[BPT][R];
 #pragma unroll
@@ -384,17 +390,51 @@ __device__ void bluestein_kernel(Lds buf, const RingFftPlan &P, int sign)
   bluestein_fft<NT, E, 1>(buf, P, -1);
 }
 
+// The passes of a transform whose length is the class size S, known at
+// compile time: in the classes of 1024 and more entries, every plan of a
+// power-of-two radix set has that length (a power-of-two N in (S/2, S] is S;
+// a Bluestein M is a power of two above the next smaller class), and its
+// radices are those fft_plan lists (engine.cpp), which the host checks
+// before the launch (fixed_fft_plan_ok).  With the lengths and strides
+// constant, the index arithmetic and the range checks of the passes fold
+// away.
+template <int S, bool BLUE> struct FixedPasses { static constexpr int n = 0, r[5] = {}; };
+template <> struct FixedPasses<8192, true> { static constexpr int n = 4, r[5] = {8, 16, 8, 8}; };
+template <> struct FixedPasses<4096, true> { static constexpr int n = 4, r[5] = {8, 8, 8, 8}; };
+template <> struct FixedPasses<2048, true> { static constexpr int n = 4, r[5] = {8, 8, 4, 8}; };
+template <> struct FixedPasses<1024, true> { static constexpr int n = 4, r[5] = {8, 8, 2, 8}; };
+template <> struct FixedPasses<8192, false> { static constexpr int n = 5, r[5] = {8, 8, 8, 8, 2}; };
+template <> struct FixedPasses<4096, false> { static constexpr int n = 4, r[5] = {8, 8, 8, 8}; };
+template <> struct FixedPasses<2048, false> { static constexpr int n = 4, r[5] = {8, 8, 8, 4}; };
+template <> struct FixedPasses<1024, false> { static constexpr int n = 4, r[5] = {8, 8, 8, 2}; };
+template <int S, int P2> constexpr bool kFixedLen = S >= 1024 && P2 >= 1;
+
+// passes I.. of the fixed plan, starting at stride NS; Bluestein transforms
+// apply `pre` in their first pass and `post` in their last
+template <int NT, int E, int S, bool BLUE, int I, int NS, typename Pre, typename Post>
+__device__ __forceinline__ void fixed_passes(Lds buf, int sign, const Pre &pre, const Post &post)
+{
+  typedef FixedPasses<S, BLUE> FP;
+  if constexpr (I < FP::n) {
+    constexpr int R = FP::r[I];
+    stockham_pass<NT, E, R, BLUE && I == 0, BLUE && I == FP::n - 1>(buf, S, NS, sign, pre, post);
+    fixed_passes<NT, E, S, BLUE, I + 1, NS * R>(buf, sign, pre, post);
+  }
+}
+
 // length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein;
 // Bt is the plan's block of the Bluestein table: the spectrum of the
 // convolution kernel (M entries, see bluestein_kernel), then the chirp
 // e^{-i pi j^2/N} (N entries)
-template <int NT, int E, bool BLUE, int P2>
+template <int NT, int E, int S, bool BLUE, int P2>
 __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
+  constexpr bool FIX = kFixedLen<S, P2>;
   if constexpr (!BLUE) {
-    lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
+    if constexpr (FIX) fixed_passes<NT, E, S, false, 0, 1>(buf, sign, PlainLoad(), PlainStore());
+    else lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
   } else {
-  const int N = P.N, M = P.M;
+  const int N = P.N, M = FIX ? S : P.M;
   const double2 *Ct = Bt + M;
   auto ch = [&](int j) { const double2 c = Ct[j]; return sign > 0 ? conj(c) : c; };
   // forward transform of the chirp-weighted input, zero-padded to M (read
@@ -410,7 +450,8 @@ __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign
     v = cmul(v, b);
     return true;
   };
-  bluestein_fft<NT, E, P2>(buf, P, -1, chirped, kernel);
+  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1>(buf, -1, chirped, kernel);
+  else bluestein_fft<NT, E, P2>(buf, P, -1, chirped, kernel);
   // inverse transform; only outputs j < N are kept, times chirp and 1/M
   const double inv = 1.0 / M;
   auto out = [&](int j, double2 &v) {
@@ -418,7 +459,8 @@ __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign
     v = cscale(cmul(v, ch(j)), inv);
     return true;
   };
-  bluestein_fft<NT, E, P2>(buf, P, +1, PlainLoad(), out);
+  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1>(buf, +1, PlainLoad(), out);
+  else bluestein_fft<NT, E, P2>(buf, P, +1, PlainLoad(), out);
   }
 }
 
@@ -679,7 +721,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
       __syncthreads();
     }
     // 3. inverse FFT of length N
-    ring_dft<NT, E, BLUE, P2>(buf, P, +1, Bt);
+    ring_dft<NT, E, S, BLUE, P2>(buf, P, +1, Bt);
     // 4. store
     void *mp = a.maps.p[comp];
     const bool add = (a.flags & SHARP_ADD) != 0;
@@ -753,7 +795,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
         [&](int j, double2 v) { buf[j] = v; });
     __syncthreads();
     // 2. forward FFT
-    ring_dft<NT, E, BLUE, P2>(buf, P, -1, Bt);
+    ring_dft<NT, E, S, BLUE, P2>(buf, P, -1, Bt);
     // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
     //    (e^{-2 pi i (N-k)/n} = -e^{2 pi i k/n} since n = 2N)
     if (even) {
@@ -847,6 +889,18 @@ template <typename K> static void set_lds(K kern, size_t lds)
 
 static size_t plan_need(const RingFftPlan &p) { return p.bluestein ? p.M : p.N; }
 
+// A plan run by the fixed passes of its class must have the class's length
+// and the passes' radices (fft_plan builds exactly these; this guards the
+// two against drifting apart).
+template <int S, bool BLUE> static bool fixed_fft_plan_ok(const RingFftPlan &p)
+{
+  typedef FixedPasses<S, BLUE> FP;
+  if ((BLUE ? p.M : p.N) != S || p.nrad != FP::n) return false;
+  for (int q = 0; q < FP::n; ++q)
+    if (p.rad[q] != FP::r[q]) return false;
+  return true;
+}
+
 template <int S, int P2>
 static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
                          const RingFftPlan *plans, const RingFftPlan *hplans, int nplans,
@@ -860,6 +914,11 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   const dim3 block(kThreads<S>);
   size_t need = 0;
   for (int i = 0; i < nplans; ++i) need = std::max(need, plan_need(hplans[i]));
+  if constexpr (kFixedLen<S, P2>) {
+    for (int i = 0; i < nplans; ++i)
+      if (!(blue ? fixed_fft_plan_ok<S, true>(hplans[i]) : fixed_fft_plan_ok<S, false>(hplans[i])))
+        SHP_FAIL("ring FFT: a plan does not match the fixed passes of its class");
+  }
   // the staging area V starts after the class's S-entry buffer (padded)
   const size_t fbytes = (size_t)lds_pad(staged ? S : (int)need) * sizeof(double2);
   // staged synthesis keeps up to vcap phases next to the FFT buffer: the
