@@ -693,6 +693,7 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
     if (!cls[c2].empty()) gp->fft[c2] = upload(cls[c2]);
     gp->hfft[c2] = cls[c2];
   }
+  if (gp->npix_fft > 0) ensure_twiddles(c.stream);
   if (btot > 0) {
     HIP_CHECK(hipMalloc(&gp->btab, (size_t)btot * sizeof(double2)));
     std::vector<RingFftPlan *> dev(kFftKinds * kNumFftClasses, nullptr);

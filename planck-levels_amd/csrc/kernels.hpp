@@ -181,6 +181,8 @@ struct FftClasses {
 };
 // Fills btab at the offsets of the given plans (one plan per distinct (N, M)).
 void launch_bluestein_table(const FftClasses &uniq, double2 *btab, hipStream_t st);
+// writes the ring FFTs' table of roots of unity on the current device (once)
+void ensure_twiddles(hipStream_t st);
 void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st);
 
 // Rings too long for every LDS class: one complex FFT of the ring's length n

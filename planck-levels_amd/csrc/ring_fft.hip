@@ -76,6 +76,13 @@ __device__ inline double2 expipi(double x)
   return expi_small(3.141592653589793 * x);
 }
 
+// The roots of unity e^{2 pi i j / kTwN}, written once per device
+// (ensure_twiddles): the twiddles of the passes with compile-time strides
+// (every Ns*R divides kTwN) and of the even-ring packing of the
+// power-of-two classes (n divides kTwN) are table reads instead of sincos.
+constexpr int kTwN = 16384;
+__device__ double2 g_twiddle[kTwN];
+
 template <typename T> __device__ inline double ld(const void *p, int64_t i)
 {
   return (double)static_cast<const T *>(p)[i];
@@ -258,8 +265,9 @@ struct PlainStore {
   __device__ bool operator()(int, double2 &) const { return true; }
 };
 
+// TWT: Ns*R divides kTwN and the twiddles come from g_twiddle
 template <int NT, int E, int R, bool FIRST = false, bool LAST = false,
-          typename Pre = PlainLoad, typename Post = PlainStore>
+          typename Pre = PlainLoad, typename Post = PlainStore, bool TWT = false>
 __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
                                      const Pre &pre = Pre(), const Post &post = Post())
 {
@@ -273,10 +281,16 @@ __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
   asm volatile("" : "+v"(tid));
   const int nb = N / R;
   double2 v[BPT][R];
+  // table twiddles are read ahead of the barrier (their latency hides
+  // behind the LDS reads and the wait)
+  double2 wt[TWT ? BPT : 1];
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int j = tid + i * NT;
     if (j < nb) {
+      if constexpr (TWT) {
+        if (Ns > 1) wt[i] = g_twiddle[(j & (Ns - 1)) * (kTwN / (Ns * R))];
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if constexpr (FIRST) v[i][r] = pre(buf, j + r * nb);
@@ -298,7 +312,9 @@ __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
     if (j < nb) {
       const int k = pow2 ? (j & (Ns - 1)) : (j - Ns * (int)(((float)j + 0.5f) * rns));
       if (Ns > 1) {
-        const double2 w = expipi((double)k * tscale);
+        double2 w;
+        if constexpr (TWT) w = sign > 0 ? wt[i] : conj(wt[i]);
+        else w = expipi((double)k * tscale);
         double2 wr = w;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -409,6 +425,19 @@ template <> struct FixedPasses<2048, false> { static constexpr int n = 4, r[5] =
 template <> struct FixedPasses<1024, false> { static constexpr int n = 4, r[5] = {8, 8, 8, 2}; };
 template <int S, int P2> constexpr bool kFixedLen = S >= 1024 && P2 >= 1;
 
+// e^{2 pi i k / n} of the even-ring packing and unpacking: a table read in
+// the power-of-two classes, where n = 2N = 2S
+template <int S, int P2, bool BLUE>
+__device__ inline double2 pack_twiddle(int k, int n)
+{
+  if constexpr (kFixedLen<S, P2> && !BLUE) {
+    static_assert(kTwN % (2 * S) == 0, "twiddle table too short");
+    return g_twiddle[k * (kTwN / (2 * S))];
+  } else {
+    return expipi((double)k * (2.0 / n));
+  }
+}
+
 // passes I.. of the fixed plan, starting at stride NS; Bluestein transforms
 // apply `pre` in their first pass and `post` in their last
 template <int NT, int E, int S, bool BLUE, int I, int NS, typename Pre, typename Post>
@@ -417,7 +446,9 @@ __device__ __forceinline__ void fixed_passes(Lds buf, int sign, const Pre &pre, 
   typedef FixedPasses<S, BLUE> FP;
   if constexpr (I < FP::n) {
     constexpr int R = FP::r[I];
-    stockham_pass<NT, E, R, BLUE && I == 0, BLUE && I == FP::n - 1>(buf, S, NS, sign, pre, post);
+    static_assert(kTwN % (NS * R) == 0, "twiddle table too short");
+    stockham_pass<NT, E, R, BLUE && I == 0, BLUE && I == FP::n - 1, Pre, Post, true>(
+        buf, S, NS, sign, pre, post);
     fixed_passes<NT, E, S, BLUE, I + 1, NS * R>(buf, sign, pre, post);
   }
 }
@@ -584,7 +615,6 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
         // Rotations: e^{i(n-k)phi0} = rn conj(rk), e^{i(N-k)phi0} = rN
         // conj(rk), e^{i(N+k)phi0} = rN rk.
         const double2 rN = norot ? one : expi_small((double)N * r.phi0);
-        const double tsc = 2.0 / n;
         auto phase_at = [&](int m) {
           return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
         };
@@ -610,7 +640,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
               if (mb <= mmax) ck = cadd(ck, conj(cval(mb, vb[u], (k == 0) ? rn : cmul(rn, conj(rk)))));
               if (j <= mmax) cj = cval(j, vc[u], cmul(rN, conj(rk)));
               if (N + k <= mmax) cj = cadd(cj, conj(cval(N + k, vd[u], cmul(rN, rk))));
-              const double2 tw = expipi((double)k * tsc);
+              const double2 tw = pack_twiddle<S, P2, BLUE>(k, n);
               const double2 e = cadd(ck, conj(cj));
               const double2 o = cmul(csub(ck, conj(cj)), tw);
               buf[k] = make_double2(e.x - o.y, e.y + o.x);
@@ -704,11 +734,10 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
   // 2. pack (even n): Z'_k = (C_k + C*_{N-k}) + i (C_k - C*_{N-k}) e^{2 pi i k/n}
     //    (e^{2 pi i (N-k)/n} = -conj(e^{2 pi i k/n}) since n = 2N)
     if (even && !packed) {
-      const double tsc = 2.0 / n;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
         const double2 ck = buf[k], ckp = (k == 0) ? nyq : buf[kp];
-        const double2 tw = expipi((double)k * tsc);
+        const double2 tw = pack_twiddle<S, P2, BLUE>(k, n);
         const double2 e = cadd(ck, conj(ckp));
         const double2 o = cmul(csub(ck, conj(ckp)), tw);
         buf[k] = make_double2(e.x - o.y, e.y + o.x);
@@ -799,7 +828,6 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
     // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
     //    (e^{-2 pi i (N-k)/n} = -e^{2 pi i k/n} since n = 2N)
     if (even) {
-      const double tsc = 2.0 / n;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
         if (k == 0) {
@@ -807,7 +835,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
           buf[0] = make_double2(z0.x + z0.y, 0.);
           nyq = make_double2(z0.x - z0.y, 0.);
         } else {
-          const double2 tw = expipi((double)k * tsc);
+          const double2 tw = pack_twiddle<S, P2, BLUE>(k, n);
           const double2 zk = buf[k], zkp = buf[kp];
           const double2 s1 = cscale(cadd(zk, conj(zkp)), 0.5);
           const double2 d1 = cmul(cscale(csub(zk, conj(zkp)), 0.5), conj(tw));
@@ -1267,6 +1295,33 @@ static void table_class(const RingFftPlan *plans, const RingFftPlan *hplans, int
   set_lds(k_bluestein_table<S>, lds);
   hipLaunchKernelGGL(k_bluestein_table<S>, dim3(np), dim3(kThreads<S>), lds, st, plans, btab);
   check_launch("bluestein_table");
+}
+
+// the roots of unity of g_twiddle, exact angles 2 j / kTwN (in units of pi)
+__global__ void __launch_bounds__(256) k_twiddle_init()
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < kTwN) {
+    double sn, cs;
+    sincospi(2.0 * j / kTwN, &sn, &cs);
+    g_twiddle[j] = make_double2(cs, sn);
+  }
+}
+
+void ensure_twiddles(hipStream_t st)
+{
+  // once per device, before any ring FFT on it (callers hold the library's
+  // lock); finished before the caller goes on, so that transforms on other
+  // streams find the table written
+  static bool done[256] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 256) SHP_FAIL("ring FFT: device index out of range");
+  if (done[dev]) return;
+  hipLaunchKernelGGL(k_twiddle_init, dim3(kTwN / 256), dim3(256), 0, st);
+  check_launch("twiddle table");
+  HIPCHK(hipStreamSynchronize(st));
+  done[dev] = true;
 }
 
 void launch_bluestein_table(const FftClasses &u, double2 *btab, hipStream_t st)
