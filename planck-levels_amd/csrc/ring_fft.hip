@@ -265,9 +265,14 @@ struct PlainStore {
   __device__ bool operator()(int, double2 &) const { return true; }
 };
 
-// TWT: Ns*R divides kTwN and the twiddles come from g_twiddle
+// TWT: Ns*R divides kTwN and the twiddles come from g_twiddle.  HIN: the
+// inputs r >= R/2 of every butterfly are zero (the first pass of a
+// transform whose upper half is zero padding); HOUT: the outputs r >= R/2
+// are not wanted (the last pass of a transform of which only the lower half
+// is kept).
 template <int NT, int E, int R, bool FIRST = false, bool LAST = false,
-          typename Pre = PlainLoad, typename Post = PlainStore, bool TWT = false>
+          typename Pre = PlainLoad, typename Post = PlainStore, bool TWT = false,
+          bool HIN = false, bool HOUT = false>
 __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
                                      const Pre &pre = Pre(), const Post &post = Post())
 {
@@ -293,7 +298,8 @@ __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (FIRST) v[i][r] = pre(buf, j + r * nb);
+        if (HIN && 2 * r >= R) v[i][r] = make_double2(0., 0.);
+        else if constexpr (FIRST) v[i][r] = pre(buf, j + r * nb);
         else v[i][r] = buf[j + r * nb];
       }
     }
@@ -325,7 +331,7 @@ __device__ __forceinline__ void stockham_pass(Lds buf, int N, int Ns, int sign,
       Dft<R>::run(v[i], sign);
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = 0; r < (HOUT ? R / 2 : R); ++r) {
         if constexpr (LAST) {
           if (post(base + r * Ns, v[i][r])) buf[base + r * Ns] = v[i][r];
         } else {
@@ -439,17 +445,25 @@ __device__ inline double2 pack_twiddle(int k, int n)
 }
 
 // passes I.. of the fixed plan, starting at stride NS; Bluestein transforms
-// apply `pre` in their first pass and `post` in their last
-template <int NT, int E, int S, bool BLUE, int I, int NS, typename Pre, typename Post>
+// apply `pre` in their first pass and `post` in their last.  HIN: the input's
+// upper half is zero; HOUT: only the output's lower half is wanted (a
+// first pass of stride 1 reads input j + r S/R of butterfly j, and a last
+// pass of stride S/R writes output j + r S/R: in both, r >= R/2 is the upper
+// half).
+template <int NT, int E, int S, bool BLUE, int I, int NS, bool HIN = false, bool HOUT = false,
+          typename Pre, typename Post>
 __device__ __forceinline__ void fixed_passes(Lds buf, int sign, const Pre &pre, const Post &post)
 {
   typedef FixedPasses<S, BLUE> FP;
   if constexpr (I < FP::n) {
     constexpr int R = FP::r[I];
+    constexpr bool first = I == 0, last = I == FP::n - 1;
     static_assert(kTwN % (NS * R) == 0, "twiddle table too short");
-    stockham_pass<NT, E, R, BLUE && I == 0, BLUE && I == FP::n - 1, Pre, Post, true>(
-        buf, S, NS, sign, pre, post);
-    fixed_passes<NT, E, S, BLUE, I + 1, NS * R>(buf, sign, pre, post);
+    static_assert(!(HIN && first) || NS == 1, "zero upper half needs the stride-1 first pass");
+    static_assert(!(HOUT && last) || NS * R == S, "lower half needs the last pass");
+    stockham_pass<NT, E, R, BLUE && first, BLUE && last, Pre, Post, true, HIN && first,
+                  HOUT && last>(buf, S, NS, sign, pre, post);
+    fixed_passes<NT, E, S, BLUE, I + 1, NS * R, HIN, HOUT>(buf, sign, pre, post);
   }
 }
 
@@ -481,7 +495,9 @@ __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign
     v = cmul(v, b);
     return true;
   };
-  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1>(buf, -1, chirped, kernel);
+  // (a Bluestein length is at least 2N: the chirped input fills at most the
+  // lower half, and only the lower half of the inverse is kept)
+  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1, true, false>(buf, -1, chirped, kernel);
   else bluestein_fft<NT, E, P2>(buf, P, -1, chirped, kernel);
   // inverse transform; only outputs j < N are kept, times chirp and 1/M
   const double inv = 1.0 / M;
@@ -490,7 +506,7 @@ __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign
     v = cscale(cmul(v, ch(j)), inv);
     return true;
   };
-  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1>(buf, +1, PlainLoad(), out);
+  if constexpr (FIX) fixed_passes<NT, E, S, true, 0, 1, false, true>(buf, +1, PlainLoad(), out);
   else bluestein_fft<NT, E, P2>(buf, P, +1, PlainLoad(), out);
   }
 }
