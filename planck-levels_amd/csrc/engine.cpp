@@ -1738,7 +1738,9 @@ static int batch_tmax(int kind, int contr)
   const bool spin0 = (kind == kBatchSynth0 || kind == kBatchAnal0);
   const bool synth = kind <= kBatchSynthD;
   if (contr != kContrF64) return spin0 ? kMixMaxT0 : kMixMaxTS;
-  if (synth) return spin0 ? 4 : 2;
+  // the FP64 matrix-core syntheses take the single-job groups (two blocks of
+  // 16 real columns); the vector-ALU ones (SHARP_HIP_FP64_SYNTHESIS=valu) 4 / 2
+  if (synth) return fp64_synth_valu() ? (spin0 ? 4 : 2) : (spin0 ? kF64SynT0 : kF64SynTS);
   // the batch FP64 analysis kernels hold one block of 16 columns
   return spin0 ? kF64MaxT0 : 4;
 }
@@ -1779,8 +1781,10 @@ static void build_work(BatchPlan &b, int ncu)
       int t0 = 0;
       while (t0 < j.ntrans) {
         int ntl = std::min(tg, j.ntrans - t0);
-        // the FP64 syntheses have kernels for power-of-two groups only
-        if (synth && b.contr == kContrF64)
+        // the vector-ALU FP64 syntheses have kernels for power-of-two groups
+        // only
+        const bool valu64 = synth && b.contr == kContrF64 && fp64_synth_valu();
+        if (valu64)
           while (ntl & (ntl - 1)) ntl &= ntl - 1;
         if (synth) {
           // the 8 orders m of a phase block (whose phases share cache lines)
@@ -1789,7 +1793,7 @@ static void build_work(BatchPlan &b, int ncu)
           const int nmb = (j.mmax + MB) / MB, nsb = (nslots + 63) / 64;
           for (int mb = 0; mb < nmb; ++mb) {
             Group g;
-            g.tl = b.contr == kContrF64 ? ntl : 0;
+            g.tl = valu64 ? ntl : 0;
             g.cost = (double)(j.lmax - mb * MB + 1) * MB * nsb * (s.c0 + s.c1 * ntl);
             for (int sb = 0; sb < nsb; ++sb)
               for (int h = 0; h < per; ++h)
@@ -1841,7 +1845,10 @@ static void build_work(BatchPlan &b, int ncu)
       BatchWork w;
       w.kind = kind;
       w.tg = tlv;
-      w.ncb = (b.contr != kContrF64 && maxntl * (spin0 ? 2 : 4) > 16) ? 2 : 1;
+      // blocks of 16 real columns: the mixed kernels and the FP64 matrix-core
+      // syntheses take one or two
+      const bool blocks = b.contr != kContrF64 || (synth && tlv == 0);
+      w.ncb = (blocks && maxntl * (spin0 ? 2 : 4) > 16) ? 2 : 1;
       w.nitems = (int)all.size();
       w.grid = std::max(8, ((tl.wg_per_cu * ncu + 7) / 8) * 8);
       if (synth) {

@@ -94,6 +94,11 @@ void launch_batch(const BatchLaunch &l, const LegArgs *jobs, hipStream_t st);
 void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArgs *jobs,
                             const BatchItem *items, const int *xofs, int *ctr, int grid,
                             int flat_grid, hipStream_t st);
+// the same items on the FP64 matrix cores (synth64 tiles, ncb blocks of 16
+// real columns)
+void launch_batch_synth64(int kind, int ncb, int shape, bool flat, const LegArgs *jobs,
+                          const BatchItem *items, const int *xofs, int *ctr, int grid,
+                          int flat_grid, hipStream_t st);
 
 // ntg = transforms in this launch (1, 2 or 4 for spin 0; 1 or 2 for spin s)
 void launch_synth(const LegArgs &a, int ntg, hipStream_t st);

@@ -857,6 +857,44 @@ __global__ void __launch_bounds__(64 * MB) kb_synth_spin(const LegArgs *__restri
   }
 }
 
+// The matrix-core FP64 syntheses over a batch's items (the tiles of
+// synth64_*_tile: NCB blocks of 16 real columns, any number of transforms up
+// to them), coefficient and a_lm rows read through the constant address space
+template <int NCB, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_synth64_spin0(const LegArgs *__restrict__ jobs,
+                                                            const BatchItem *__restrict__ items,
+                                                            const int *__restrict__ xofs,
+                                                            int *__restrict__ ctr)
+{
+  auto tile = [&](const BatchItem &it) {
+    synth64_spin0_tile<NCB, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(), it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
+  }
+}
+
+template <int NCB, bool DERIV, bool FLAT>
+__global__ void __launch_bounds__(64 * MB) kb_synth64_spin(const LegArgs *__restrict__ jobs,
+                                                           const BatchItem *__restrict__ items,
+                                                           const int *__restrict__ xofs,
+                                                           int *__restrict__ ctr)
+{
+  auto tile = [&](const BatchItem &it) {
+    synth64_spin_tile<NCB, DERIV, true>(job_args(jobs, it), it.major * blockDim.y + wave_y(),
+                                        it.minor);
+  };
+  if constexpr (FLAT) {
+    for_flat_item(items, xofs, tile);
+  } else {
+    __shared__ int fetched;
+    for_group_items(items, xofs, ctr, fetched, tile);
+  }
+}
+
 // ===========================================================================
 // spin 0 analysis
 // ===========================================================================
@@ -1231,6 +1269,23 @@ static void batch_synth_f64(int kind, int tg, dim3 b, int grid, int flat_grid, h
   }
 }
 
+template <bool FLAT>
+static void batch_synth64(int kind, int ncb, dim3 b, int grid, int flat_grid, hipStream_t st,
+                          const LegArgs *jobs, const BatchItem *items, const int *xofs, int *ctr)
+{
+  auto go = [&](auto kernel) {
+    launch_batch_kernel(kernel, FLAT, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  };
+  const bool two = ncb == 2;
+  if (kind == kBatchSynth0) {
+    if (two) go(kb_synth64_spin0<2, FLAT>); else go(kb_synth64_spin0<1, FLAT>);
+  } else if (kind == kBatchSynthS) {
+    if (two) go(kb_synth64_spin<2, false, FLAT>); else go(kb_synth64_spin<1, false, FLAT>);
+  } else {
+    if (two) go(kb_synth64_spin<2, true, FLAT>); else go(kb_synth64_spin<1, true, FLAT>);
+  }
+}
+
 void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArgs *jobs,
                             const BatchItem *items, const int *xofs, int *ctr, int grid,
                             int flat_grid, hipStream_t st)
@@ -1239,6 +1294,17 @@ void launch_batch_synth_f64(int kind, int tg, int shape, bool flat, const LegArg
   if (flat) batch_synth_f64<true>(kind, tg, b, grid, flat_grid, st, jobs, items, xofs, ctr);
   else batch_synth_f64<false>(kind, tg, b, grid, flat_grid, st, jobs, items, xofs, ctr);
   check_launch("batched FP64 synthesis");
+}
+
+void launch_batch_synth64(int kind, int ncb, int shape, bool flat, const LegArgs *jobs,
+                          const BatchItem *items, const int *xofs, int *ctr, int grid,
+                          int flat_grid, hipStream_t st)
+{
+  if (ncb != 1 && ncb != 2) SHP_FAIL("bad column block count");
+  const dim3 b(64, shape);
+  if (flat) batch_synth64<true>(kind, ncb, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  else batch_synth64<false>(kind, ncb, b, grid, flat_grid, st, jobs, items, xofs, ctr);
+  check_launch("batched FP64 matrix-core synthesis");
 }
 
 void launch_anal(const LegArgs &a, int ntg, hipStream_t st)

@@ -1728,7 +1728,13 @@ void launch_batch(const BatchLaunch &L, const LegArgs *jobs, hipStream_t st)
     // shape: wavefronts per workgroup, one order m each
     if (shape != 4 && shape != 8) SHP_FAIL("bad batch synthesis workgroup");
     if (L.contr == kContrF64) {
-      launch_batch_synth_f64(kind, L.tg, shape, L.flat, jobs, L.items, L.xofs, L.ctr, L.grid,
+      // tg > 0: the vector-ALU kernels specialised by transform group;
+      // tg == 0: the matrix-core kernels
+      if (L.tg > 0)
+        launch_batch_synth_f64(kind, L.tg, shape, L.flat, jobs, L.items, L.xofs, L.ctr, L.grid,
+                               L.flat_grid, st);
+      else
+        launch_batch_synth64(kind, L.ncb, shape, L.flat, jobs, L.items, L.xofs, L.ctr, L.grid,
                              L.flat_grid, st);
       return;
     }
