@@ -1171,6 +1171,7 @@ static JobState setup_job(Ctx &c, const Job &j, void *const *dalm, void *const *
   aa.bsplit = nullptr;
   aa.bs_base = nullptr;
   aa.bs_ncol = 0;
+  aa.bs_deriv = j.deriv ? 1 : 0;
   aa.spin = j.spin;
   aa.part = nullptr;
   aa.gmax = ml.gmax;
@@ -1520,7 +1521,7 @@ static bool uses_bsplit(const Job &j)
   return is_mixed(j.flags) && mixed_contr(j.flags) == kContrBf16x3 && j.spin > 0 &&
          j.type == SHARP_ALM2MAP;
 }
-static int bs_ncol(const Job &j) { return 2 * j.ntrans * j.nalm + 1; }
+static int bs_ncol(const Job &j) { return 4 * j.ntrans + 1; }
 // 32-bit words of the BF16 pieces (the layout of bs_pairs)
 static int64_t bs_words(const Job &j)
 {

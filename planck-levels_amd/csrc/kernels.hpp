@@ -49,11 +49,18 @@ struct LegArgs {
 };
 
 // BF16 pieces of the FP32 a_lm of a BF16 spin-s synthesis, written by the
-// a_lm gather: for each m, real column c (2*ncomp_alm of them, then one zero
-// column) and piece q (h, m, l of the exact three-piece split) the rows
-// l = o, o+1, ... from o = max(m, spin) as pairs (o+2r, o+2r+1) in one 32-bit
-// word (row o+2r in the low half, the row past lmax zero), bs_pairs words
-// per (c, q), starting at word bs_base[m] + (3c + q) * bs_pairs.
+// a_lm gather as the B operand words of the synthesis's first accumulator.
+// Output column c = 4t+k is (Q.re, Q.im, U.re, U.im)[k] of transform t; its
+// lambda_w entry is X_c = (E.re, E.im, B.re, B.im)[k] and its lambda_x entry
+// Y_c = s_k X_(c^3), s_k = -1 for k = 0, 3 and +1 else (libsharp/
+// sharp_core_inc2.c:282-306; without B, as for DERIV1, X = (E.re, E.im, 0, 0)).
+// For each m, column c (4*ntrans of them, then one zero column) and piece q
+// (h, m, l of the exact three-piece split) the rows l = o, o+1, ... from
+// o = max(m, spin) form pairs r: one 32-bit word holds X_c of row o+2r (low
+// half) and Y_c of row o+2r+1 (high half), zero past lmax; bs_pairs words per
+// (c, q), starting at word bs_base[m] + (3c + q) * bs_pairs.  The second
+// accumulator's word (Y_c of row o+2r, X_c of row o+2r+1) is s_k times the
+// word of column c^3.
 __host__ __device__ inline int bs_pairs(int lmax, int o)
 {
   const int p = (lmax - o + 2) / 2;
@@ -247,6 +254,7 @@ struct AlmArgs {
   unsigned *bsplit;
   const int64_t *bs_base;
   int bs_ncol, spin;
+  int bs_deriv;         // the a_lm are E only (DERIV1): X = (E.re, E.im, 0, 0)
 };
 // dst[c][k] = src[c][k] - dst[c][k] for k < n, c < ncomp (float or double)
 void launch_map_residual(const PtrTab &src, const PtrTab &dst, int ncomp, int64_t n,

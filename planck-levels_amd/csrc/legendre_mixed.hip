@@ -683,34 +683,25 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb, double2 *cst)
           flush(tot1, c1);
           flush(tot2, c2);
           nblk = 0;
-          // B from the a_lm split into BF16 pieces by the gather (bs_pairs):
-          // X and Y columns of this lane (the zero column for unused ones), Y
-          // negated by its sign bit; rows l0+8kq.. are the 4 words from pair
-          // (l0+8kq-mhi)/2 (l - mhi is even here)
+          // B from the piece words the gather wrote per output column
+          // (bs_pairs): accumulator 1 takes the words of this lane's column
+          // c, accumulator 2 those of column c^3, whose products carry the
+          // sign s_k of column c (added to the total as s_k times the group
+          // sum); unused columns read the zero column.  Rows l0+8kq.. are the
+          // 4 words from pair (l0+8kq-mhi)/2 (l - mhi is even here).
           const SplitK sk = split_consts();
           const unsigned *bsm = a.bsplit + a.bs_base[m];
           const int np = bs_pairs(lmax, st.mhi);
-          const int czero = a.bs_ncol - 1, cbase = a.t0 * NALM * 2;
+          const int czero = a.bs_ncol - 1, cbase = 4 * a.t0;
           const unsigned *px[NCB], *py[NCB];
-          unsigned sgy[NCB];
+          float s2[NCB];
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb) {
-            const int col = cb * 16 + n, t = col >> 2, q = col & 3;
+            const int col = cb * 16 + n, q = col & 3;
             const bool ok = col < ncol;
-            int cx, cy;
-            bool neg;
-            if (!DERIV) {
-              cx = ok ? cbase + col : czero;
-              cy = ok ? cbase + (col ^ 3) : czero;
-              neg = ok && (q == 0 || q == 3);
-            } else {
-              cx = (ok && q < 2) ? cbase + 2 * t + q : czero;
-              cy = (ok && q >= 2) ? cbase + 2 * t + (q == 2 ? 1 : 0) : czero;
-              neg = ok && q == 3;
-            }
-            px[cb] = bsm + (int64_t)3 * cx * np + 4 * kq;
-            py[cb] = bsm + (int64_t)3 * cy * np + 4 * kq;
-            sgy[cb] = neg ? 0x80008000u : 0u;
+            px[cb] = bsm + (int64_t)3 * (ok ? cbase + col : czero) * np + 4 * kq;
+            py[cb] = bsm + (int64_t)3 * (ok ? cbase + (col ^ 3) : czero) * np + 4 * kq;
+            s2[cb] = (q == 0 || q == 3) ? -1.f : 1.f;
           }
           // Recurrence coefficients (f0, g) of l0+1 .. l0+32 in this
           // wavefront's LDS slice, staged one group ahead: lanes 0..31 load
@@ -777,26 +768,14 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb, double2 *cst)
               for (int r = 0; r < 4; ++r) { a1[8 * r + p] = t1[r]; a2[8 * r + p] = t2[r]; }
             }
             // accumulator 1: (X row 2d, Y row 2d+1) per word, accumulator 2:
-            // (Y row 2d, X row 2d+1), as split3 of the FP32 rows would give;
-            // both halves of the Y words take the sign first, then each word
-            // is one byte permute (low half of the first, high half of the
-            // second operand word)
+            // (Y row 2d, X row 2d+1) = s_k times the words of column c^3
             Bf3 B1[NCB], B2[NCB];
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
-              u32x4 w1[3], w2[3];
-#pragma unroll
-              for (int q = 0; q < 3; ++q)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                  const unsigned x = PX[cb][q][e], y = PY[cb][q][e] ^ sgy[cb];
-                  w1[q][e] = __builtin_amdgcn_perm(y, x, 0x07060100u);
-                  w2[q][e] = __builtin_amdgcn_perm(x, y, 0x07060100u);
-                }
-              B1[cb] = Bf3{__builtin_bit_cast(bf16x8, w1[0]), __builtin_bit_cast(bf16x8, w1[1]),
-                           __builtin_bit_cast(bf16x8, w1[2])};
-              B2[cb] = Bf3{__builtin_bit_cast(bf16x8, w2[0]), __builtin_bit_cast(bf16x8, w2[1]),
-                           __builtin_bit_cast(bf16x8, w2[2])};
+              B1[cb] = Bf3{__builtin_bit_cast(bf16x8, PX[cb][0]), __builtin_bit_cast(bf16x8, PX[cb][1]),
+                           __builtin_bit_cast(bf16x8, PX[cb][2])};
+              B2[cb] = Bf3{__builtin_bit_cast(bf16x8, PY[cb][0]), __builtin_bit_cast(bf16x8, PY[cb][1]),
+                           __builtin_bit_cast(bf16x8, PY[cb][2])};
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -809,7 +788,9 @@ __device__ void msynth_spin_tile(const LegArgs &a, int m, int sb, double2 *cst)
 #pragma unroll
               for (int cb = 0; cb < NCB; ++cb) {
                 tot1[cb][r] += mfma_bf3(A1, B1[cb], f4{0.f, 0.f, 0.f, 0.f});
-                tot2[cb][r] += mfma_bf3(A2, B2[cb], f4{0.f, 0.f, 0.f, 0.f});
+                const f4 g2 = mfma_bf3(A2, B2[cb], f4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+                for (int v = 0; v < 4; ++v) tot2[cb][r][v] = fmaf(s2[cb], g2[v], tot2[cb][r][v]);
               }
             }
             cbi ^= 1;

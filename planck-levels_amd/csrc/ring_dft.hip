@@ -301,12 +301,13 @@ __global__ void __launch_bounds__(256) k_alm_gather(AlmArgs a)
 }
 
 // Gather of a BF16 spin-s synthesis: the FP32 canonical rows as k_alm_gather
-// writes them, and their BF16 pieces (bs_pairs).  One workgroup per (m, 64
-// row pairs from o = max(m, spin), 16 components) goes through an LDS tile,
-// so that the reads of the caller's a_lm (along l), the FP32 rows (along the
-// columns) and the piece words (along the pairs) are all coalesced.  The
-// rows m <= l < o (m < spin) get FP32 rows only; column 2*ncomp of the pieces
-// is the zero column.
+// writes them, and the BF16 piece words of the output columns (bs_pairs).
+// One workgroup per (m, 64 row pairs from o = max(m, spin), 16 components)
+// goes through an LDS tile, so that the reads of the caller's a_lm (along l),
+// the FP32 rows (along the columns) and the piece words (along the pairs) are
+// all coalesced; 16 components are 8 whole transforms (E, B) or 16 (E only),
+// so the columns c and c^3 of a word are in one tile.  The rows m <= l < o
+// (m < spin) get FP32 rows only; the last output column is the zero column.
 template <typename T>
 __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
 {
@@ -327,8 +328,7 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
     }
   }
   if (r0 >= np) return;
-  // real columns of this chunk; the last chunk also writes the zero column
-  const int nr = min(64, np - r0), ncol = min(32, ncf - 2 * c0);
+  const int nr = min(64, np - r0), ncomp = min(16, nc - c0);
   const int lb = o0 + 2 * r0;
   for (int e = threadIdx.x; e < 128 * 16; e += 256) {
     const int i = e & 127, cc = e >> 7, l = lb + i, comp = c0 + cc;
@@ -343,10 +343,19 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
     if (i < 2 * nr && l <= a.lmax && c < ncf) rows[(int64_t)l * ncf + c] = tile[i][cc];
   }
   unsigned *w = a.bsplit + a.bs_base[m];
-  for (int e = threadIdx.x; e < ncol * 3 * 64; e += 256) {
-    const int p = e & 63, q = (e >> 6) % 3, cc = (e >> 6) / 3;
+  // output columns of this chunk: 4 per transform, from column ocb on
+  const int nout = a.bs_deriv ? 4 * ncomp : 2 * ncomp;
+  const int ocb = a.bs_deriv ? 4 * c0 : 2 * c0;
+  // X of output column k (tile row i), or 0
+  auto xval = [&](int i, int k) {
+    if (!a.bs_deriv) return tile[i][k];
+    return (k & 3) < 2 ? tile[i][2 * (k >> 2) + (k & 3)] : 0.f;
+  };
+  for (int e = threadIdx.x; e < nout * 3 * 64; e += 256) {
+    const int p = e & 63, q = (e >> 6) % 3, k = (e >> 6) / 3;
     if (p >= nr) continue;
-    // piece q of the exact three-piece split (x = h + m + l, truncation)
+    // piece q of the exact three-piece split (x = h + m + l, truncation);
+    // the split of -x is the negated split of x
     auto piece = [&](float x) {
       const unsigned xu = __float_as_uint(x);
       if (q == 0) return xu >> 16;
@@ -355,10 +364,12 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
       if (q == 1) return ru >> 16;
       return __float_as_uint(r1 - __uint_as_float(ru & 0xffff0000u)) >> 16;
     };
-    w[(int64_t)(3 * (2 * c0 + cc) + q) * np + r0 + p] =
-        (piece(tile[2 * p + 1][cc]) << 16) | piece(tile[2 * p][cc]);
+    const int kk = k & 3;
+    const float sy = (kk == 0 || kk == 3) ? -1.f : 1.f;
+    const float x = xval(2 * p, k), y = sy * xval(2 * p + 1, k ^ 3);
+    w[(int64_t)(3 * (ocb + k) + q) * np + r0 + p] = (piece(y) << 16) | piece(x);
   }
-  if (2 * c0 + 32 >= ncf) {
+  if (c0 + 16 >= nc) {
     const int zc = a.bs_ncol - 1;
     for (int e = threadIdx.x; e < 3 * 64; e += 256) {
       const int p = e & 63, q = e >> 6;
@@ -372,8 +383,10 @@ void launch_alm_gather_split(const AlmArgs &a, hipStream_t s)
   // tiles from the longest m (pairs from max(m, spin)); chunks of 16
   // components, the last one also writing the zero column
   const int ntiles = std::max(1, (bs_pairs(a.lmax, a.spin) + 63) / 64);
-  const int nchunks = std::max(1, (2 * a.ncomp + 31) / 32);
-  if (a.bs_ncol != 2 * a.ncomp + 1) SHP_FAIL("BF16 a_lm split: bad column count");
+  const int nchunks = std::max(1, (a.ncomp + 15) / 16);
+  const int nout = a.bs_deriv ? 4 * a.ncomp : 2 * a.ncomp;
+  if (a.bs_ncol != nout + 1) SHP_FAIL("BF16 a_lm split: bad column count");
+  if (!a.bs_deriv && (a.ncomp & 1)) SHP_FAIL("BF16 a_lm split: E and B per transform expected");
   const dim3 grid(ntiles * nchunks, a.mmax + 1);
   if (a.flags & SHARP_DP)
     hipLaunchKernelGGL(k_alm_gather_split<double>, grid, dim3(256), 0, s, a, ntiles);
