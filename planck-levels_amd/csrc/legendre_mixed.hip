@@ -906,19 +906,32 @@ __device__ inline void slice_row8(const float *sl, int ll, int c0, float (&v)[8]
 }
 
 // Adds the wavefronts' tiles (in slice 0 of each wavefront) in a fixed order
-// and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm.
+// and stores rows lc+x+2i, columns cb*16+n of the group's partial a_lm; a
+// thread takes 4 adjacent columns of one row (16-byte LDS reads, one vector
+// store where all 4 are columns of the launch).
 template <typename T, int NT>
 __device__ inline void reduce_store(T *const *slices, int nw, T *prow, int64_t ld, int lc,
                                     int lmax, int ncol)
 {
-  for (int e = threadIdx.x; e < NT * 256; e += blockDim.x) {
-    const int tile = e >> 8, i = (e >> 4) & 15, nn = e & 15;
+  for (int e = threadIdx.x; e < NT * 64; e += blockDim.x) {
+    const int tile = e >> 6, i = (e >> 2) & 15, nn = 4 * (e & 3);
     const int off = (tile * 16 + i) * kTS + nn;
-    T sum = 0;
-    for (int ww = 0; ww < nw; ++ww) sum += slices[ww][off];
+    T sum[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sum[k] = slices[0][off + k];
+    for (int ww = 1; ww < nw; ++ww)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sum[k] += slices[ww][off + k];
     const int x = tile & 1, cb = tile >> 1;
     const int ll = lc + x + 2 * i, col = cb * 16 + nn;
-    if (ll <= lmax && col < ncol) prow[(int64_t)ll * ld + col] = sum;
+    if (ll > lmax || col >= ncol) continue;
+    T *dst = prow + (int64_t)ll * ld + col;
+    if (col + 3 < ncol) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) dst[k] = sum[k];
+    } else {
+      for (int k = 0; k < ncol - col; ++k) dst[k] = sum[k];
+    }
   }
 }
 
