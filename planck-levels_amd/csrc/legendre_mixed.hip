@@ -892,6 +892,12 @@ constexpr int kTS = 20;    // LDS row stride of the [16 x 16] reduction tiles
 constexpr int kRS = 68;
 // floats per wavefront slice: the larger of the two layouts
 constexpr int kSlice = (32 * kRS > 64 * kLS) ? 32 * kRS : 64 * kLS;
+// BF16-split spin-s analysis: lambda_w and lambda_x of a ring pair as one
+// 8-byte pair, [32 l][64 ring pairs][w, x] rows of kRS2 floats in the two
+// slices of a wavefront (one 8-byte store per l and lane, conflict-free; 16
+// lanes on 16 rows read 16 bytes each on distinct banks at 4 mod 64)
+constexpr int kRS2 = 132;
+static_assert(32 * kRS2 <= 2 * kSlice, "paired lambda rows do not fit two slices");
 
 // eight lambda values of row ll, ring pairs c0..c0+7, of a [l][ring] slice
 __device__ inline void slice_row8(const float *sl, int ll, int c0, float (&v)[8])
@@ -902,6 +908,21 @@ __device__ inline void slice_row8(const float *sl, int ll, int c0, float (&v)[8]
     const float2 t = p[q];
     v[2 * q] = t.x;
     v[2 * q + 1] = t.y;
+  }
+}
+
+// eight (lambda_w, lambda_x) pairs of row ll, ring pairs c0..c0+7, of a
+// paired [l][ring][w, x] slice
+__device__ inline void slice_row8_wx(const float *sl, int ll, int c0, float (&w)[8], float (&x)[8])
+{
+  const float4 *p = reinterpret_cast<const float4 *>(sl + ll * kRS2 + 2 * c0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 t = p[q];
+    w[2 * q] = t.x;
+    x[2 * q] = t.y;
+    w[2 * q + 1] = t.z;
+    x[2 * q + 1] = t.w;
   }
 }
 
@@ -1018,8 +1039,18 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
         bq[1][kb][cb] = split3(v2, sk);
       }
   }
-  // slice index of local row ll (0..31) of this lane's ring pair
-  auto sidx = [&](int ll) { return CT ? ll * kRS + lane : lane * kLS + ll; };
+  // local rows 2j, 2j+1 (0..31) of this lane's ring pair: as one 8-byte
+  // pair in row j of a [16][64 ring pairs][2] slice (BF16 splits; rows of
+  // kRS2 floats), else at lane * kLS + row
+  auto put = [&](int j, T v0, T v1) {
+    if constexpr (CT == 1) {
+      *reinterpret_cast<float2 *>(lds[w] + j * kRS2 + 2 * lane) = make_float2(v0, v1);
+    } else {
+      lds[w][lane * kLS + 2 * j] = v0;
+      lds[w][lane * kLS + 2 * j + 1] = v1;
+    }
+  };
+  static_assert(16 * kRS2 <= kSlice, "paired lambda rows do not fit the slice");
 
   const double cth = a.cth[s], sth = a.sth[s];
   const double2 *rf = reinterpret_cast<const double2 *>(a.tab) + (tab_ofs(m, lmax) - m);
@@ -1079,8 +1110,7 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
         } else {
           lam2 = (cth * lam1) * cq[4 * q + 2] - lam2 * cq[4 * q + 3];
         }
-        mine[sidx(2 * q)] = (T)o0;
-        mine[sidx(2 * q + 1)] = (T)o1;
+        put(q, (T)o0, (T)o1);
       }
       l += 32;
     } else {
@@ -1121,8 +1151,7 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mine[sidx(2 * j)] = active ? (T)o0 : (T)0;
-        mine[sidx(2 * j + 1)] = active ? (T)o1 : (T)0;
+        put(j, active ? (T)o0 : (T)0, active ? (T)o1 : (T)0);
         l += 2;
         if (q == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -1159,24 +1188,19 @@ __device__ void manal_spin0_tile(const LegArgs &a, int m, int g,
           }
       } else {
         // A: rows x+2n (l = lc+x+2n), ring pairs 32kb+8kq..+7, for the
-        // groups q = (x, kb) in turn; the LDS reads and the split of group
-        // q+1 are interleaved with the matrix products of group q
+        // groups q = (kb, x) in turn (one read gives both x of a kb); the
+        // LDS reads and the split of group q+1 are interleaved with the
+        // matrix products of group q
         const float *sm = reinterpret_cast<const float *>(mine);
-        Bf3 A;
-        {
-          float av[8];
-          slice_row8(sm, 2 * n, 8 * kq, av);
-          A = split3(av, sk);
-        }
+        float a0[8], a1[8];
+        slice_row8_wx(sm, n, 8 * kq, a0, a1);
+        Bf3 A = split3(a0, sk);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int x = q >> 1, kb = q & 1;
+          const int kb = q >> 1, x = q & 1;
           Bf3 nA = A;
-          if (q < 3) {
-            float av[8];
-            slice_row8(sm, ((q + 1) >> 1) + 2 * n, 32 * ((q + 1) & 1) + 8 * kq, av);
-            nA = split3(av, sk);
-          }
+          if (q == 1) slice_row8_wx(sm, n, 32 + 8 * kq, a0, a1);
+          if (q < 3) nA = split3((q & 1) ? a0 : a1, sk);
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb) t[x][cb] = mfma_bf3(A, bq[x][kb][cb], t[x][cb]);
           if (q < 3) sched_interleave<6 * NCB, 28 / (6 * NCB) + 1>();
@@ -1293,6 +1317,15 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   double cfp = corfac(scalep), cfm = corfac(scalem);
   int l = st.mhi;
   T *mw = lds[w][0], *mx = lds[w][1];
+  // lambda_w, lambda_x of row ll of this lane's ring pair
+  auto put = [&](int ll, T wv, T xv) {
+    if constexpr (CT == 1) {
+      *reinterpret_cast<float2 *>(mw + ll * kRS2 + 2 * lane) = make_float2(wv, xv);
+    } else {
+      mw[sidx(ll)] = wv;
+      mx[sidx(ll)] = xv;
+    }
+  };
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
@@ -1353,10 +1386,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         if constexpr (sizeof(T) == 4) {
           // combinations in FP32 of the FP32-rounded chains
           const float p0 = (float)rp0, m0 = (float)rm0, p1 = (float)rp1, m1 = (float)rm1;
-          mw[sidx(2 * j)] = p0 + m0;
-          mw[sidx(2 * j + 1)] = p1 + m1;
-          mx[sidx(2 * j)] = m0 - p0;
-          mx[sidx(2 * j + 1)] = m1 - p1;
+          put(2 * j, p0 + m0, m0 - p0);
+          put(2 * j + 1, p1 + m1, m1 - p1);
         } else {
           mw[sidx(2 * j)] = (T)(rp0 + rm0);
           mw[sidx(2 * j + 1)] = (T)(rp1 + rm1);
@@ -1412,10 +1443,8 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
             }
           }
         }
-        mw[sidx(2 * j)] = active ? (T)lw0 : (T)0;
-        mw[sidx(2 * j + 1)] = active ? (T)lw1 : (T)0;
-        mx[sidx(2 * j)] = active ? (T)lx0 : (T)0;
-        mx[sidx(2 * j + 1)] = active ? (T)lx1 : (T)0;
+        put(2 * j, active ? (T)lw0 : (T)0, active ? (T)lx0 : (T)0);
+        put(2 * j + 1, active ? (T)lw1 : (T)0, active ? (T)lx1 : (T)0);
         l += 2;
         if (h == 0) {
           __builtin_amdgcn_sched_barrier(0);
@@ -1455,12 +1484,10 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
         // groups q = (x, kb) in turn; the LDS reads and the splits of group
         // q+1 are interleaved with the matrix products of group q
         const float *sw = reinterpret_cast<const float *>(mw);
-        const float *sx = reinterpret_cast<const float *>(mx);
         Bf3 Aw, Ax;
         {
           float aw[8], ax[8];
-          slice_row8(sw, 2 * n, 8 * kq, aw);
-          slice_row8(sx, 2 * n, 8 * kq, ax);
+          slice_row8_wx(sw, 2 * n, 8 * kq, aw, ax);
           Aw = split3(aw, sk);
           Ax = split3(ax, sk);
         }
@@ -1471,8 +1498,7 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
           if (q < 3) {
             const int x1 = (q + 1) >> 1, kb1 = (q + 1) & 1;
             float aw[8], ax[8];
-            slice_row8(sw, x1 + 2 * n, 32 * kb1 + 8 * kq, aw);
-            slice_row8(sx, x1 + 2 * n, 32 * kb1 + 8 * kq, ax);
+            slice_row8_wx(sw, x1 + 2 * n, 32 * kb1 + 8 * kq, aw, ax);
             nw = split3(aw, sk);
             nx = split3(ax, sk);
           }
