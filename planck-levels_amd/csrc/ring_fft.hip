@@ -76,6 +76,17 @@ __device__ inline double2 expipi(double x)
   return expi_small(3.141592653589793 * x);
 }
 
+// a wave-uniform value in scalar registers (the compiler keeps FP64 values
+// computed by vector instructions in vector registers otherwise)
+__device__ inline double uniform(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double2 uniform(double2 v) { return make_double2(uniform(v.x), uniform(v.y)); }
+
 // The roots of unity e^{2 pi i j / kTwN}, written once per device
 // (ensure_twiddles): the twiddles of the passes with compile-time strides
 // (every Ns*R divides kTwN) and of the even-ring packing of the
@@ -538,9 +549,9 @@ template <int S> constexpr int kThreads = (S / 8 < 256) ? 256 : ((S / 8 < 512) ?
 // Runs consume(i, load(i)) for i = lo+tid, lo+tid+NT, ... < hi with U loads
 // in flight per thread (all U are issued before the first is used).
 template <int NT, int U, typename L, typename C>
-__device__ inline void batched(int lo, int hi, L &&load, C &&consume)
+__device__ inline void batched(int tid, int lo, int hi, L &&load, C &&consume)
 {
-  for (int base = lo + (int)threadIdx.x; base < hi; base += NT * U) {
+  for (int base = lo + tid; base < hi; base += NT * U) {
     double2 v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -598,8 +609,19 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
   };
   const double2 one = make_double2(1., 0.);
   // rotation steps: e^{i NT phi0} between a thread's bins, e^{i n phi0}
-  const double2 rstep = norot ? one : expi_small((double)NT * r.phi0);
-  const double2 rn = norot ? one : expi_small((double)n * r.phi0);
+  const double2 rstep = uniform(norot ? one : expi_small((double)NT * r.phi0));
+  const double2 rn = uniform(norot ? one : expi_small((double)n * r.phi0));
+  // the same for every component (the fused fold and pack of even rings):
+  // e^{i N phi0}; e^{i k phi0} of the thread's first bin k = tid and the
+  // step 2 NT to its next pair of bins; the packing twiddle e^{2 pi i k/n}
+  // of the first bin and the step NT between bins (where n is not a table
+  // length).  Chained by products, they keep sine and cosine evaluations out
+  // of the component loop.
+  const double2 rN = uniform(norot ? one : expi_small((double)N * r.phi0));
+  const double2 rk0 = norot ? one : expi_small((double)tid * r.phi0);
+  const double2 rstep2 = uniform(norot ? one : expi_small((double)(2 * NT) * r.phi0));
+  const double2 tw0 = expipi((double)tid * (2.0 / n));
+  const double2 twstep = uniform(expipi((double)NT * (2.0 / n)));
 
   // bins: even n -> C_0..C_N, odd n -> C_0..C_{(n-1)/2}
   const int nbins = even ? N + 1 : (n + 1) / 2;
@@ -630,11 +652,11 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
         // (step 2 below), with no LDS round trip between the two steps.
         // Rotations: e^{i(n-k)phi0} = rn conj(rk), e^{i(N-k)phi0} = rN
         // conj(rk), e^{i(N+k)phi0} = rN rk.
-        const double2 rN = norot ? one : expi_small((double)N * r.phi0);
         auto phase_at = [&](int m) {
           return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
         };
         const double2 zero = make_double2(0., 0.);
+        double2 rkb = rk0, twk = tw0;
         for (int base = tid; 2 * base <= N; base += 2 * NT) {
           double2 va[2], vb[2], vc[2], vd[2];
 #pragma unroll
@@ -646,17 +668,18 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
             vc[u] = (ok && j <= mmax) ? phase_at(j) : zero;
             vd[u] = (ok && N + k <= mmax) ? phase_at(N + k) : zero;
           }
-          double2 rk = norot ? one : expi_small((double)base * r.phi0);
+          double2 rk = rkb;
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const int k = base + u * NT, j = N - k, mb = (k == 0) ? n : n - k;
+            const double2 tw = (kFixedLen<S, P2> && !BLUE) ? pack_twiddle<S, P2, BLUE>(k, n) : twk;
+            twk = cmul(twk, twstep);
             if (2 * k <= N) {
               double2 ck = zero, cj = zero;
               if (k <= mmax) ck = cval(k, va[u], rk);
               if (mb <= mmax) ck = cadd(ck, conj(cval(mb, vb[u], (k == 0) ? rn : cmul(rn, conj(rk)))));
               if (j <= mmax) cj = cval(j, vc[u], cmul(rN, conj(rk)));
               if (N + k <= mmax) cj = cadd(cj, conj(cval(N + k, vd[u], cmul(rN, rk))));
-              const double2 tw = pack_twiddle<S, P2, BLUE>(k, n);
               const double2 e = cadd(ck, conj(cj));
               const double2 o = cmul(csub(ck, conj(cj)), tw);
               buf[k] = make_double2(e.x - o.y, e.y + o.x);
@@ -668,6 +691,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
             }
             rk = cmul(rk, rstep);
           }
+          rkb = cmul(rkb, rstep2);
         }
         packed = true;
       } else
@@ -818,7 +842,17 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
 
   const double2 *Bt = BLUE ? btab + P.boff : nullptr;
   const double2 one = make_double2(1., 0.);
-  const double2 rstep = norot ? one : expi_small((double)NT * (-r.phi0));
+  // the same for every component: e^{-i m phi0} of the thread's first order
+  // m = tid and the steps NT (between its orders) and 8 NT (a product with
+  // the step of 8 orders every 8 orders keeps the chain short); the unpacking
+  // twiddle e^{2 pi i k/n} of the first bin k = tid and the step NT (where n
+  // is not a table length).  Chained by products, they keep sine and cosine
+  // evaluations out of the component loop.
+  const double2 rot0 = norot ? one : expi_small((double)tid * (-r.phi0));
+  const double2 rstep = uniform(norot ? one : expi_small((double)NT * (-r.phi0)));
+  const double2 rstep8 = uniform(norot ? one : expi_small((double)(8 * NT) * (-r.phi0)));
+  const double2 tw0 = expipi((double)tid * (2.0 / n));
+  const double2 twstep = uniform(expipi((double)NT * (2.0 / n)));
 
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
@@ -827,7 +861,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
     // pixel pairs as one vector access: unit stride, pair start aligned
     const bool pairs = even && pair_aligned<T>(mp, r.ofs - a.map_base, r.stride);
     // 1. load: even n -> z_j = f_2j + i f_2j+1, odd n -> z_j = f_j
-    batched<NT, (P2 == 2) ? 4 : 8>(0, N,
+    batched<NT, (P2 == 2) ? 4 : 8>(tid, 0, N,
         [&](int j) {
           if (even) {
             const int64_t i0 = r.ofs + (int64_t)(2 * j) * r.stride - a.map_base;
@@ -844,14 +878,16 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
     // 3. unpack (even n): X_k = (Z_k + Z*_{N-k})/2 - i/2 e^{-2 pi i k/n} (Z_k - Z*_{N-k})
     //    (e^{-2 pi i (N-k)/n} = -e^{2 pi i k/n} since n = 2N)
     if (even) {
+      double2 twk = tw0;
       for (int k = tid; 2 * k <= N; k += NT) {
         const int kp = N - k;
+        const double2 tw = (kFixedLen<S, P2> && !BLUE) ? pack_twiddle<S, P2, BLUE>(k, n) : twk;
+        twk = cmul(twk, twstep);
         if (k == 0) {
           const double2 z0 = buf[0];
           buf[0] = make_double2(z0.x + z0.y, 0.);
           nyq = make_double2(z0.x - z0.y, 0.);
         } else {
-          const double2 tw = pack_twiddle<S, P2, BLUE>(k, n);
           const double2 zk = buf[k], zkp = buf[kp];
           const double2 s1 = cscale(cadd(zk, conj(zkp)), 0.5);
           const double2 d1 = cmul(cscale(csub(zk, conj(zkp)), 0.5), conj(tw));
@@ -867,7 +903,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
       __syncthreads();
     }
     // 4. phases
-    double2 rot = one;
+    double2 rot = rot0, rot8 = rot0;
     int step = 0;
     for (int m = tid; m <= mmax; m += NT, ++step) {
       const int k = m % n;
@@ -876,8 +912,12 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
       else v = buf[k];
       v = cscale(v, wgt);
       if (!norot) {
-        // e^{-i m phi0}: one sincos every 8 steps, complex products between
-        rot = ((step & 7) == 0) ? expi_small((double)m * (-r.phi0)) : cmul(rot, rstep);
+        // e^{-i m phi0}: products by the step of 8 orders every 8 orders,
+        // by the step of one order between
+        if (step > 0) {
+          if ((step & 7) == 0) rot = rot8 = cmul(rot8, rstep8);
+          else rot = cmul(rot, rstep);
+        }
         v = cmul(v, rot);
       }
       a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)] = v;
