@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "kernels.hpp"
 #include "sharp_lowlevel.h"
 
@@ -415,21 +416,51 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
   if (a.part) {
     const int64_t gstride = a.part_rows * a.part_ld;
     const int64_t o = (row - a.part_row0) * a.part_ld + 2 * (comp - a.part_c0);
-    double re = 0., im = 0.;
-    for (int g = 0; g < a.ngroups; ++g) {
-      if (a.gmax[g] < m) continue;
-      if (a.part_f64) {
-        const double2 v = *reinterpret_cast<const double2 *>(
-            static_cast<const double *>(a.part) + o + g * gstride);
-        re += v.x;
-        im += v.y;
-      } else {
-        const float2 v = *reinterpret_cast<const float2 *>(
-            static_cast<const float *>(a.part) + o + g * gstride);
-        re += v.x;
-        im += v.y;
+    auto part_at = [&](auto f64, int g) {
+      if constexpr (decltype(f64)::value)
+        return *reinterpret_cast<const double2 *>(static_cast<const double *>(a.part) + o +
+                                                  g * gstride);
+      else {
+        const float2 v = *reinterpret_cast<const float2 *>(static_cast<const float *>(a.part) +
+                                                           o + g * gstride);
+        return make_double2(v.x, v.y);
       }
+    };
+    // the groups that reach m (gmax[g] >= m), added in group order.  When
+    // gmax does not decrease along the groups (ring pairs ordered from a
+    // pole towards the equator, as in every iso-latitude grid) they are the
+    // groups from the first one that reaches m on: their loads are issued
+    // four at a time, not one after the other behind each test
+    int glo = 0;
+    bool mono = true;
+    for (int g = 0; g < a.ngroups; ++g) {
+      const int gm = a.gmax[g];
+      glo += gm < m;
+      if (g > 0) mono = mono && gm >= a.gmax[g - 1];
     }
+    double re = 0., im = 0.;
+    auto sum = [&](auto f64) {
+      if (mono) {
+        // (indices past the last group read the last group again, unused)
+        for (int g0 = glo; g0 < a.ngroups; g0 += 4) {
+          double2 v[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = part_at(f64, min(g0 + i, a.ngroups - 1));
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (g0 + i < a.ngroups) { re += v[i].x; im += v[i].y; }
+        }
+      } else {
+        for (int g = 0; g < a.ngroups; ++g) {
+          if (a.gmax[g] < m) continue;
+          const double2 v = part_at(f64, g);
+          re += v.x;
+          im += v.y;
+        }
+      }
+    };
+    if (a.part_f64) sum(std::true_type());
+    else sum(std::false_type());
     x = make_double2(re, im);
   } else {
     x = a.almtmp[row * a.ncomp + comp];
