@@ -440,6 +440,18 @@ struct Ctx {
   // them and may run concurrently
   struct StreamBufs {
     DevBuf phase, almtmp, part, resid, lfac, bsplit, lscratch;
+    // the ring FFT stage's two extra streams (created on first use)
+    FftFork fork{};
+    bool has_fork = false;
+    ~StreamBufs()
+    {
+      if (!has_fork) return;
+      for (int i = 0; i < 2; ++i) {
+        (void)hipStreamDestroy(fork.aux[i]);
+        (void)hipEventDestroy(fork.join[i]);
+      }
+      (void)hipEventDestroy(fork.fork);
+    }
   };
   std::map<hipStream_t, std::unique_ptr<StreamBufs>> sbufs;
   StreamBufs &bufs(hipStream_t st)
@@ -1058,6 +1070,8 @@ static void timed(Ctx &c, hipStream_t st, const char *name, double flops,
 // every stage and the algorithmic work of each stage (for the profiler).
 struct JobState {
   Job j;
+  // the ring FFT stage's extra streams, or nullptr: one stream
+  const FftFork *fk = nullptr;
   GeomPlan *gp = nullptr;
   GeomPlan::Mlim *ml = nullptr;
   AlmArgs aa{};
@@ -1302,7 +1316,7 @@ static void stage_rings_to_phase(Ctx &c, JobState &s, hipStream_t st)
           [&] { launch_ring2phase_dft(s.ra, gp.all_list, gp.nall, st); });
   } else {
     timed(c, st, "ring_fft_anal", 0., s.ring_bytes_fft,
-          [&] { launch_ring_fft(false, s.ra, s.fc, st); });
+          [&] { launch_ring_fft(false, s.ra, s.fc, st, s.fk); });
     if (gp.nlong)
       timed(c, st, "ring_long_anal", 0., s.ring_bytes_long,
             [&] { launch_long_rings(false, s.ra, long_args(s), s.lslots, st); });
@@ -1320,7 +1334,7 @@ static void stage_phase_to_rings(Ctx &c, JobState &s, hipStream_t st)
           [&] { launch_phase2ring_dft(s.ra, gp.all_list, gp.nall, st); });
   } else {
     timed(c, st, "ring_fft_synth", 0., s.ring_bytes_fft,
-          [&] { launch_ring_fft(true, s.ra, s.fc, st); });
+          [&] { launch_ring_fft(true, s.ra, s.fc, st, s.fk); });
     if (gp.nlong)
       timed(c, st, "ring_long_synth", 0., s.ring_bytes_long,
             [&] { launch_long_rings(true, s.ra, long_args(s), s.lslots, st); });
@@ -1531,6 +1545,25 @@ static int64_t bs_words(const Job &j)
   return t;
 }
 
+// The extra streams of the ring FFT stage on launch stream st, created with
+// st's device current on first use; nullptr (one stream) while st is being
+// captured into a graph.
+static const FftFork *ring_fork(Ctx::StreamBufs &b, hipStream_t st)
+{
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_CHECK(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return nullptr;
+  if (!b.has_fork) {
+    for (int i = 0; i < 2; ++i) {
+      HIP_CHECK(hipStreamCreateWithFlags(&b.fork.aux[i], hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreateWithFlags(&b.fork.join[i], hipEventDisableTiming));
+    }
+    HIP_CHECK(hipEventCreateWithFlags(&b.fork.fork, hipEventDisableTiming));
+    b.has_fork = true;
+  }
+  return &b.fork;
+}
+
 static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dmap,
                        int64_t alm_base, int64_t map_base, hipStream_t st,
                        unsigned long long *opcnt, const WorkPtrs *w = nullptr)
@@ -1544,6 +1577,7 @@ static void run_device(Ctx &c, const Job &j, void *const *dalm, void *const *dma
                       : static_cast<double2 *>(
                             sb->almtmp.get(almtmp_elems(j) * sizeof(double2), st, true));
   JobState s = setup_job(c, j, dalm, dmap, alm_base, map_base, phase, almtmp, opcnt, st);
+  if (!w) s.fk = ring_fork(*sb, st);
   if (w) {
     s.lslots = std::min(w->lslots, long_slots(gp, j));
     s.lscratch = w->lscratch;
@@ -2535,7 +2569,8 @@ void sharp_hip_release(void)
   c.bsplans.clear();
   for (auto &kv : c.sbufs) {
     Ctx::StreamBufs &b = *kv.second;
-    for (DevBuf *d : {&b.phase, &b.almtmp, &b.part, &b.resid, &b.lfac, &b.bsplit}) d->release();
+    for (DevBuf *d : {&b.phase, &b.almtmp, &b.part, &b.resid, &b.lfac, &b.bsplit, &b.lscratch})
+      d->release();
   }
   c.sbufs.clear();
   c.stage_alm.release();

@@ -195,7 +195,16 @@ struct FftClasses {
 void launch_bluestein_table(const FftClasses &uniq, double2 *btab, hipStream_t st);
 // writes the ring FFTs' table of roots of unity on the current device (once)
 void ensure_twiddles(hipStream_t st);
-void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st);
+// Two more streams of the launch stream's device and the events that fork
+// them from it and join them back: with them the ring FFT classes of one
+// stage run on three streams, so that a class's last workgroups overlap the
+// next classes' first ones instead of leaving CUs idle.
+struct FftFork {
+  hipStream_t aux[2];
+  hipEvent_t fork, join[2];
+};
+void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st,
+                     const FftFork *fk = nullptr);
 
 // Rings too long for every LDS class: one complex FFT of the ring's length n
 // through global memory (power-of-two n), or Bluestein with L = 2^lg >= 2n-1.

@@ -1321,13 +1321,23 @@ void launch_long_table(const LongRingPlan *plans, int nplans, double2 *scratch, 
   check_launch("long-ring Bluestein table");
 }
 
-void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st)
+void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStream_t st0,
+                     const FftFork *fk)
 {
+  // (fk) the classes go round-robin over st0 and fk's two streams, which
+  // start after everything before on st0 and which st0 waits for at the end
+  if (fk) {
+    HIPCHK(hipEventRecord(fk->fork, st0));
+    for (int i = 0; i < 2; ++i) HIPCHK(hipStreamWaitEvent(fk->aux[i], fk->fork, 0));
+  }
+  int next = 0;
   // largest rings first: they take longest
   for (int c = kNumFftClasses - 1; c >= 0; --c) {
     for (int kind = kFftKinds - 1; kind >= 0; --kind) {
       const int idx = kFftKinds * c + kind;
       if (fc.count[idx] == 0) continue;
+      const hipStream_t st = (!fk || next == 0) ? st0 : fk->aux[next - 1];
+      next = (next + 1) % 3;
       const RingFftPlan *p = fc.plans[idx], *hp = fc.hplans[idx];
       const int np = fc.count[idx];
       const bool blue = (kind & 1) != 0;
@@ -1338,6 +1348,11 @@ void launch_ring_fft(bool synth, const RingArgs &a, const FftClasses &fc, hipStr
       }
     }
   }
+  if (fk)
+    for (int i = 0; i < 2; ++i) {
+      HIPCHK(hipEventRecord(fk->join[i], fk->aux[i]));
+      HIPCHK(hipStreamWaitEvent(st0, fk->join[i], 0));
+    }
 }
 
 template <int S>
