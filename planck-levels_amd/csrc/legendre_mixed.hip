@@ -1329,6 +1329,51 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   T *slices[W];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) slices[ww] = lds[ww][0];
+
+  // Whole chunks below the IEEE range for every wavefront of the workgroup
+  // (mode 0, or idle): their lambda are all zero, so their recurrence runs
+  // here without stores, products or reduction, and their partial rows are
+  // written as zeros.  The workgroup votes after each chunk (two barriers);
+  // a chunk that any wavefront leaves in range is rolled back and processed
+  // by the chunk loop.  The recurrence steps are those of the chunk loop's
+  // mode 0, so the state it continues from is the same.
+  const int nchunks = (lmax - st.mhi + 32) / 32;
+  int c0 = 0;
+  {
+    __shared__ int vote[2];
+    if (threadIdx.x == 0) vote[0] = vote[1] = 1;
+    __syncthreads();
+    for (; st.mhi + 32 * c0 + 31 <= lmax; ++c0) {
+      const int lc = st.mhi + 32 * c0;
+      const double s1p = rec1p, s1m = rec1m, s2p = rec2p, s2m = rec2m;
+      const int ssp = scalep, ssm = scalem;
+      bool stay = mode == 3;
+      if (mode == 0) {
+        for (int k = 0; k < 32; k += 2) {
+          rec_step(rec1p, rec1m, rec2p, rec2m, cth, fx(lc + 1 + k));
+          rec_step(rec2p, rec2m, rec1p, rec1m, cth, fx(lc + 2 + k));
+          rescale1(rec1p, rec2p, scalep);
+          rescale1(rec1m, rec2m, scalem);
+        }
+        stay = wave_all(scalep < 1 && scalem < 1);
+      }
+      if (!stay && lane == 0) vote[c0 & 1] = 0;
+      __syncthreads();
+      const bool all = vote[c0 & 1] != 0;
+      if (threadIdx.x == 0) vote[(c0 + 1) & 1] = 1;
+      __syncthreads();
+      if (!all) {
+        rec1p = s1p; rec1m = s1m; rec2p = s2p; rec2m = s2m;
+        scalep = ssp; scalem = ssm;
+        break;
+      }
+    }
+    const int nz = 32 * c0 * ncol;
+    for (int e = threadIdx.x; e < nz; e += blockDim.x)
+      prow[(int64_t)(st.mhi + e / ncol) * a.part_ld + e % ncol] = (T)0;
+  }
+  l = st.mhi + 32 * c0;
+
   // coefficients fx(lc+1 .. lc+32) of a chunk (96 doubles) in LDS, shared
   // by the workgroup's wavefronts (one m) and staged one chunk ahead; the
   // unscaled chunks read them with wave-uniform addresses
@@ -1336,12 +1381,11 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
   auto stage_coef = [&](int lc, double *dst) {
     if (threadIdx.x < 96) dst[threadIdx.x] = fxb[min(3 * (lc + 1) + (int)threadIdx.x, fdmax)];
   };
-  stage_coef(st.mhi, cbuf[0]);
+  stage_coef(l, cbuf[c0 & 1]);
   __syncthreads();
   Fx fc[4];
 
-  const int nchunks = (lmax - st.mhi + 32) / 32;
-  for (int c = 0; c < nchunks; ++c) {
+  for (int c = c0; c < nchunks; ++c) {
     const int lc = st.mhi + 32 * c;
     // the other buffer was last read in chunk c-1, before its barriers
     stage_coef(lc + 32, cbuf[(c + 1) & 1]);
