@@ -1387,8 +1387,11 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
 
   for (int c = c0; c < nchunks; ++c) {
     const int lc = st.mhi + 32 * c;
-    // the other buffer was last read in chunk c-1, before its barriers
-    stage_coef(lc + 32, cbuf[(c + 1) & 1]);
+    // the next chunk's coefficients: loaded here, stored into the other
+    // buffer (last read in chunk c-1, before its barriers) at the end of the
+    // chunk, so that the load's latency is not waited for up front
+    const double cnext =
+        threadIdx.x < 96 ? fxb[min(3 * (lc + 33) + (int)threadIdx.x, fdmax)] : 0.;
     // the buffer index as a VGPR value (an identity DPP move, which the
     // compiler does not take as uniform): the reads below then address the
     // buffer as one VGPR plus immediate offsets, not as a scalar address
@@ -1576,6 +1579,7 @@ __device__ void manal_spin_tile(const LegArgs &a, int m, int g,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     write_tiles<T, NCB>(mw, tw);
+    if (threadIdx.x < 96) cbuf[(c + 1) & 1][threadIdx.x] = cnext;
     __syncthreads();
     reduce_store<T, 2 * NCB>(slices, W, prow, a.part_ld, lc, lmax, ncol);
     __syncthreads();
