@@ -690,7 +690,9 @@ static GeomPlan &get_geom(Ctx &c, const sharp_geom_info *g)
       auto it = boffs.find(key);
       if (it == boffs.end()) {
         it = boffs.emplace(key, btot).first;
-        btot += pl.M + pl.N;   // kernel spectrum, then the chirp
+        // kernel spectrum, then the chirp: the halves 0..M/2 and 0..N/2 of
+        // the two symmetric sequences
+        btot += pl.M / 2 + 1 + pl.N / 2 + 1;
         RingFftPlan u = pl;
         u.boff = it->second;
         ucls[(c2 / kFftKinds) * kFftKinds + 1].push_back(u);

@@ -177,7 +177,8 @@ struct RingFftPlan {
   int nrad;
   int8_t rad[16];
   int64_t boff;    // Bluestein: offset of the plan's block in the table
-                   // (M kernel-spectrum entries, then N chirp entries)
+                   // (kernel-spectrum entries 0..M/2, then chirp entries
+                   // 0..N/2: the halves of two symmetric sequences)
 };
 constexpr int kNumFftClasses = 5;   // LDS buffers of 512 << c complex doubles
 // kinds of plan in a class: bluestein + 2 * radix set (0: 8, 4, 2, 3, 5, 7;

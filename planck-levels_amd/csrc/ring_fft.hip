@@ -480,8 +480,8 @@ __device__ __forceinline__ void fixed_passes(Lds buf, int sign, const Pre &pre, 
 
 // length-N transform of buf[0..N) with the sign `sign`, smooth or Bluestein;
 // Bt is the plan's block of the Bluestein table: the spectrum of the
-// convolution kernel (M entries, see bluestein_kernel), then the chirp
-// e^{-i pi j^2/N} (N entries)
+// convolution kernel (entries 0..M/2, see bluestein_kernel), then the chirp
+// e^{-i pi j^2/N} (entries 0..N/2)
 template <int NT, int E, int S, bool BLUE, int P2>
 __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign, const double2 *Bt)
 {
@@ -491,19 +491,27 @@ __device__ __forceinline__ void ring_dft(Lds buf, const RingFftPlan &P, int sign
     else lds_fft<NT, E, P2>(buf, P.N, P.rad, P.nrad, sign);
   } else {
   const int N = P.N, M = FIX ? S : P.M;
-  const double2 *Ct = Bt + M;
-  auto ch = [&](int j) { const double2 c = Ct[j]; return sign > 0 ? conj(c) : c; };
+  // the table holds half of each symmetric sequence: the kernel spectrum
+  // B_k = B_{M-k} for k <= M/2, the chirp c_j = (-1)^N c_{N-j} for j <= N/2
+  const double2 *Ct = Bt + M / 2 + 1;
+  auto ch = [&](int j) {
+    const bool hi = 2 * j > N;
+    double2 c = Ct[hi ? N - j : j];
+    if (hi && (N & 1)) c = make_double2(-c.x, -c.y);
+    return sign > 0 ? conj(c) : c;
+  };
   // forward transform of the chirp-weighted input, zero-padded to M (read
   // in its first pass), times the kernel spectrum (in its last pass).  The
   // kernel of sign s is the conjugate of the kernel of sign -s with the
   // input conjugated: FFT(conj(x))_k = conj(FFT(x)_{-k}); the table holds
-  // the spectrum for the analysis sign -1
+  // the spectrum for the analysis sign -1, and as B_{-k} = B_k the other
+  // sign's factor is conj(B_k)
   auto chirped = [&](Lds b, int j) {
     return j < N ? cmul(b[j], ch(j)) : make_double2(0., 0.);
   };
   auto kernel = [&](int j, double2 &v) {
-    const double2 b = sign > 0 ? conj(Bt[j == 0 ? 0 : M - j]) : Bt[j];
-    v = cmul(v, b);
+    const double2 b = Bt[2 * j > M ? M - j : j];
+    v = cmul(v, sign > 0 ? conj(b) : b);
     return true;
   };
   // (a Bluestein length is at least 2N: the chirped input fills at most the
@@ -956,8 +964,10 @@ __global__ void __launch_bounds__(kThreads<S>) k_bluestein_table(const RingFftPl
   const RingFftPlan &P = plans[blockIdx.x];
   const Lds buf{smem};
   bluestein_kernel<NT, E>(buf, P, -1);
-  for (int j = threadIdx.x; j < P.M; j += NT) btab[P.boff + j] = buf[j];
-  for (int j = threadIdx.x; j < P.N; j += NT) btab[P.boff + P.M + j] = chirp(j, P.N, -1);
+  // halves of the symmetric sequences (ring_dft): B_0..B_{M/2}, c_0..c_{N/2}
+  for (int j = threadIdx.x; 2 * j <= P.M; j += NT) btab[P.boff + j] = buf[j];
+  for (int j = threadIdx.x; 2 * j <= P.N; j += NT)
+    btab[P.boff + P.M / 2 + 1 + j] = chirp(j, P.N, -1);
 }
 
 // raises a kernel's dynamic LDS limit once per new maximum (callers hold the
