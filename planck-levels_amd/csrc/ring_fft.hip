@@ -584,13 +584,19 @@ __device__ inline void batched(int tid, int lo, int hi, L &&load, C &&consume)
 // SIMD, which caps its registers at 128 (it needs no more)
 template <int S, bool BLUE, bool STAGED, int P2>
 constexpr int kMinBlocks = (S == 4096 && !BLUE && !STAGED && P2 == 2) ? 4 : 1;
+// threads per workgroup: the unstaged 8192 Bluestein class (the polar
+// rings) runs 1024 threads, 8 entries each: its 136 KB of LDS hold one
+// workgroup per CU, and 16 waves hide more of the LDS latency of the three
+// transforms than 8 do
+template <int S, bool BLUE, bool STAGED>
+constexpr int kSynThreads = (S == 8192 && BLUE && !STAGED) ? 1024 : kThreads<S>;
 
 template <int S, bool BLUE, bool STAGED, int P2, typename T>
-__global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
+__global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S, BLUE, STAGED, P2>)) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
                                                           int nplans, int comps_per_block, int vcap,
                                                           const double2 *btab)
 {
-  constexpr int NT = kThreads<S>, E = S / NT;
+  constexpr int NT = kSynThreads<S, BLUE, STAGED>, E = S / NT;
   // buf: the ring's FFT buffer; V (STAGED): up to vcap weighted, rotated phases
   extern __shared__ double2 smem[];
   const Lds buf{smem};
@@ -599,7 +605,7 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
   const int pi = pair_plan(blockIdx.x);
   if (pi >= nplans) return;
   const RingFftPlan &P = plans[pi];
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
   const int slot = P.ring >> 1, hemi = P.ring & 1;
   const RingDev r = a.ring[P.ring];
   const int n = P.n, N = P.N, mmax = a.mmax;
@@ -649,6 +655,10 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, STAGED, P2>)
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
+    // at 1024 threads the register cap is 128: an opaque tid per component
+    // keeps the compiler from holding every tid-derived LDS address live
+    // across the loop (which spills)
+    if constexpr (NT == 1024) asm volatile("" : "+v"(tid));
     // 1. fold: C_k = sum over m = k (mod n) of c_m + conj(sum over m = -k
     //    (mod n), m > 0, of c_m)
     bool packed = false;
@@ -1005,7 +1015,6 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   int cpb = a.ncomp;
   while (cpb > 1 && (int64_t)nplans * ((a.ncomp + cpb - 1) / cpb) < 2048) cpb = (cpb + 1) / 2;
   const dim3 grid((nplans + 15) / 16 * 16, (a.ncomp + cpb - 1) / cpb);
-  const dim3 block(kThreads<S>);
   size_t need = 0;
   for (int i = 0; i < nplans; ++i) need = std::max(need, plan_need(hplans[i]));
   if constexpr (kFixedLen<S, P2>) {
@@ -1025,14 +1034,15 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
     if (vcap < 1) SHP_FAIL("ring FFT: no LDS left for the phase staging");
   }
   const size_t lds = fbytes + (size_t)vcap * sizeof(double2);
-  auto go = [&](auto kern) {
+  auto go = [&](auto kern, int nt = kThreads<S>) {
     set_lds(kern, lds);
-    hipLaunchKernelGGL(kern, grid, block, lds, st, a, plans, nplans, cpb, vcap, btab);
+    hipLaunchKernelGGL(kern, grid, dim3(nt), lds, st, a, plans, nplans, cpb, vcap, btab);
   };
 #define SHP_SYN(B, ST)                                                     \
   do {                                                                     \
-    if (a.dp) go(k_phase2ring_fft<S, B, ST, P2, double>);                  \
-    else go(k_phase2ring_fft<S, B, ST, P2, float>);                        \
+    constexpr int nt = kSynThreads<S, B, ST>;                              \
+    if (a.dp) go(k_phase2ring_fft<S, B, ST, P2, double>, nt);              \
+    else go(k_phase2ring_fft<S, B, ST, P2, float>, nt);                    \
   } while (0)
   // Bluestein lengths are powers of two: no Bluestein kernel of the radix
   // set with 3, 5, 7
