@@ -581,9 +581,11 @@ __device__ inline void batched(int tid, int lo, int hi, L &&load, C &&consume)
 // ===========================================================================
 // minimum waves per SIMD: the radix-8 (equatorial) 4096 class, unstaged,
 // runs two 512-thread workgroups per CU (72 KB of LDS each), 4 waves per
-// SIMD, which caps its registers at 128 (it needs no more)
-template <int S, bool BLUE, bool STAGED, int P2>
-constexpr int kMinBlocks = (S == 4096 && !BLUE && !STAGED && P2 == 2) ? 4 : 1;
+// SIMD, which caps its registers at 128 (it needs no more); so does the
+// 4096 Bluestein class with the gathered two-alias fold (no staging area)
+template <int S, bool BLUE, bool STAGED, int P2, int NA = 1>
+constexpr int kMinBlocks = ((S == 4096 && !BLUE && !STAGED && P2 == 2) ||
+                            (S == 4096 && BLUE && !STAGED && NA == 2)) ? 4 : 1;
 // threads per workgroup: the unstaged 8192 Bluestein class (the polar
 // rings) runs 1024 threads, 8 entries each: its 136 KB of LDS hold one
 // workgroup per CU, and 16 waves hide more of the LDS latency of the three
@@ -591,8 +593,10 @@ constexpr int kMinBlocks = (S == 4096 && !BLUE && !STAGED && P2 == 2) ? 4 : 1;
 template <int S, bool BLUE, bool STAGED>
 constexpr int kSynThreads = (S == 8192 && BLUE && !STAGED) ? 1024 : kThreads<S>;
 
-template <int S, bool BLUE, bool STAGED, int P2, typename T>
-__global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S, BLUE, STAGED, P2>)) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
+// NA (unstaged fold): most aliases m = k (mod n) of one bin in [0, mmax],
+// 1 for rings with n > mmax, 2 for mmax < 2n
+template <int S, bool BLUE, bool STAGED, int P2, typename T, int NA = 1>
+__global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S, BLUE, STAGED, P2, NA>)) k_phase2ring_fft(RingArgs a, const RingFftPlan *plans,
                                                           int nplans, int comps_per_block, int vcap,
                                                           const double2 *btab)
 {
@@ -663,41 +667,61 @@ __global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S,
     //    (mod n), m > 0, of c_m)
     bool packed = false;
     if constexpr (!STAGED) {
+      // mmax < NA n: a bin's aliases are its first NA chain entries m, m + n
       if (even) {
-        // n > mmax, even n: fold and pack in one pass.  The thread of k in
-        // [0, N/2] forms the bins C_k and C_{N-k} (each at most one phase
-        // and one conjugated alias) and writes the packed Z_k and Z_{N-k}
-        // (step 2 below), with no LDS round trip between the two steps.
-        // Rotations: e^{i(n-k)phi0} = rn conj(rk), e^{i(N-k)phi0} = rN
-        // conj(rk), e^{i(N+k)phi0} = rN rk.
+        // even n: fold and pack in one pass.  The thread of k in [0, N/2]
+        // forms the bins C_k and C_{N-k} (each from the alias chains of k and
+        // n-k, and of N-k and N+k, conjugated for the second of each) and
+        // writes the packed Z_k and Z_{N-k} (step 2 below), with no LDS round
+        // trip between the two steps.  Rotations: e^{i(n-k)phi0} = rn
+        // conj(rk), e^{i(N-k)phi0} = rN conj(rk), e^{i(N+k)phi0} = rN rk,
+        // times rn per step along a chain.
         auto phase_at = [&](int m) {
           return a.phase[phase_index(comp, m, slot, hemi, a.nslots, a.nmb)];
         };
         const double2 zero = make_double2(0., 0.);
+        // bin pairs per thread and round: two with one alias, else one
+        constexpr int U = (NA == 1) ? 2 : 1;
+        const double2 rstepU = (U == 2) ? rstep2 : rstep;
         double2 rkb = rk0, twk = tw0;
-        for (int base = tid; 2 * base <= N; base += 2 * NT) {
-          double2 va[2], vb[2], vc[2], vd[2];
+        for (int base = tid; 2 * base <= N; base += U * NT) {
+          double2 va[U][NA], vb[U][NA], vc[U][NA], vd[U][NA];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < U; ++u) {
             const int k = base + u * NT, j = N - k, mb = (k == 0) ? n : n - k;
             const bool ok = 2 * k <= N;
-            va[u] = (ok && k <= mmax) ? phase_at(k) : zero;
-            vb[u] = (ok && mb <= mmax) ? phase_at(mb) : zero;
-            vc[u] = (ok && j <= mmax) ? phase_at(j) : zero;
-            vd[u] = (ok && N + k <= mmax) ? phase_at(N + k) : zero;
+#pragma unroll
+            for (int q = 0; q < NA; ++q) {
+              const int o = q * n;
+              va[u][q] = (ok && k + o <= mmax) ? phase_at(k + o) : zero;
+              vb[u][q] = (ok && mb + o <= mmax) ? phase_at(mb + o) : zero;
+              vc[u][q] = (ok && j + o <= mmax) ? phase_at(j + o) : zero;
+              vd[u][q] = (ok && N + k + o <= mmax) ? phase_at(N + k + o) : zero;
+            }
           }
           double2 rk = rkb;
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < U; ++u) {
             const int k = base + u * NT, j = N - k, mb = (k == 0) ? n : n - k;
             const double2 tw = (kFixedLen<S, P2> && !BLUE) ? pack_twiddle<S, P2, BLUE>(k, n) : twk;
             twk = cmul(twk, twstep);
             if (2 * k <= N) {
+              double2 ra = rk, rb = (k == 0) ? rn : cmul(rn, conj(rk));
+              double2 rc = cmul(rN, conj(rk)), rd = cmul(rN, rk);
               double2 ck = zero, cj = zero;
-              if (k <= mmax) ck = cval(k, va[u], rk);
-              if (mb <= mmax) ck = cadd(ck, conj(cval(mb, vb[u], (k == 0) ? rn : cmul(rn, conj(rk)))));
-              if (j <= mmax) cj = cval(j, vc[u], cmul(rN, conj(rk)));
-              if (N + k <= mmax) cj = cadd(cj, conj(cval(N + k, vd[u], cmul(rN, rk))));
+              if (k <= mmax) ck = cval(k, va[u][0], ra);
+              if (mb <= mmax) ck = cadd(ck, conj(cval(mb, vb[u][0], rb)));
+              if (j <= mmax) cj = cval(j, vc[u][0], rc);
+              if (N + k <= mmax) cj = cadd(cj, conj(cval(N + k, vd[u][0], rd)));
+#pragma unroll
+              for (int q = 1; q < NA; ++q) {
+                const int o = q * n;
+                ra = cmul(ra, rn); rb = cmul(rb, rn); rc = cmul(rc, rn); rd = cmul(rd, rn);
+                if (k + o <= mmax) ck = cadd(ck, cval(k + o, va[u][q], ra));
+                if (mb + o <= mmax) ck = cadd(ck, conj(cval(mb + o, vb[u][q], rb)));
+                if (j + o <= mmax) cj = cadd(cj, cval(j + o, vc[u][q], rc));
+                if (N + k + o <= mmax) cj = cadd(cj, conj(cval(N + k + o, vd[u][q], rd)));
+              }
               const double2 e = cadd(ck, conj(cj));
               const double2 o = cmul(csub(ck, conj(cj)), tw);
               buf[k] = make_double2(e.x - o.y, e.y + o.x);
@@ -709,37 +733,50 @@ __global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S,
             }
             rk = cmul(rk, rstep);
           }
-          rkb = cmul(rkb, rstep2);
+          rkb = cmul(rkb, rstepU);
         }
         packed = true;
-      } else
-      // n > mmax: at most c_k and conj(c_{n-k}) per bin; 4 bins (8 loads)
-      // in flight per thread
-      for (int base = tid; base < nbins; base += 4 * NT) {
-        double2 vp[4], vn[4];
+      } else {
+        // odd n: the chains of k and n-k (conjugated) per bin; 4 bins (8
+        // loads) in flight per thread with one alias, 2 bins with two
+        constexpr int U = (NA == 1) ? 4 : 2;
+        for (int base = tid; base < nbins; base += U * NT) {
+          double2 vp[U][NA], vn[U][NA];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = base + u * NT, mn = (k == 0) ? n : n - k;
-          vp[u] = (k < nbins && k <= mmax)
-                      ? a.phase[phase_index(comp, k, slot, hemi, a.nslots, a.nmb)]
-                      : make_double2(0., 0.);
-          vn[u] = (k < nbins && mn <= mmax)
-                      ? a.phase[phase_index(comp, mn, slot, hemi, a.nslots, a.nmb)]
-                      : make_double2(0., 0.);
-        }
-        // e^{i k phi0} for this thread's bins; e^{i (n-k) phi0} from it
-        double2 rk = norot ? one : expi_small((double)base * r.phi0);
+          for (int u = 0; u < U; ++u) {
+            const int k = base + u * NT, mn = (k == 0) ? n : n - k;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = base + u * NT, mn = (k == 0) ? n : n - k;
-          if (k < nbins) {
-            const double2 rmn = (k == 0) ? rn : cmul(rn, conj(rk));
-            double2 sum = make_double2(0., 0.);
-            if (k <= mmax) sum = cval(k, vp[u], rk);
-            if (mn <= mmax) sum = cadd(sum, conj(cval(mn, vn[u], rmn)));
-            put_bin(k, sum, true);
+            for (int q = 0; q < NA; ++q) {
+              const int o = q * n;
+              vp[u][q] = (k < nbins && k + o <= mmax)
+                             ? a.phase[phase_index(comp, k + o, slot, hemi, a.nslots, a.nmb)]
+                             : make_double2(0., 0.);
+              vn[u][q] = (k < nbins && mn + o <= mmax)
+                             ? a.phase[phase_index(comp, mn + o, slot, hemi, a.nslots, a.nmb)]
+                             : make_double2(0., 0.);
+            }
           }
-          rk = cmul(rk, rstep);
+          // e^{i k phi0} for this thread's bins; e^{i (n-k) phi0} from it
+          double2 rk = norot ? one : expi_small((double)base * r.phi0);
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int k = base + u * NT, mn = (k == 0) ? n : n - k;
+            if (k < nbins) {
+              double2 rp = rk, rmn = (k == 0) ? rn : cmul(rn, conj(rk));
+              double2 sum = make_double2(0., 0.);
+              if (k <= mmax) sum = cval(k, vp[u][0], rp);
+              if (mn <= mmax) sum = cadd(sum, conj(cval(mn, vn[u][0], rmn)));
+#pragma unroll
+              for (int q = 1; q < NA; ++q) {
+                const int o = q * n;
+                rp = cmul(rp, rn); rmn = cmul(rmn, rn);
+                if (k + o <= mmax) sum = cadd(sum, cval(k + o, vp[u][q], rp));
+                if (mn + o <= mmax) sum = cadd(sum, conj(cval(mn + o, vn[u][q], rmn)));
+              }
+              put_bin(k, sum, true);
+            }
+            rk = cmul(rk, rstep);
+          }
         }
       }
       __syncthreads();
@@ -1008,7 +1045,7 @@ template <int S, bool BLUE> static bool fixed_fft_plan_ok(const RingFftPlan &p)
 template <int S, int P2>
 static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
                          const RingFftPlan *plans, const RingFftPlan *hplans, int nplans,
-                         const double2 *btab, hipStream_t st)
+                         const double2 *btab, hipStream_t st, int na = 1)
 {
   if (nplans <= 0) return;
   // split components over blocks only when there are few rings
@@ -1041,6 +1078,13 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
 #define SHP_SYN(B, ST)                                                     \
   do {                                                                     \
     constexpr int nt = kSynThreads<S, B, ST>;                              \
+    if constexpr (!ST && S >= 2048) {                                      \
+      if (na == 2) {                                                       \
+        if (a.dp) go(k_phase2ring_fft<S, B, false, P2, double, 2>, nt);    \
+        else go(k_phase2ring_fft<S, B, false, P2, float, 2>, nt);          \
+        break;                                                             \
+      }                                                                    \
+    }                                                                      \
     if (a.dp) go(k_phase2ring_fft<S, B, ST, P2, double>, nt);              \
     else go(k_phase2ring_fft<S, B, ST, P2, float>, nt);                    \
   } while (0)
@@ -1085,11 +1129,18 @@ static void launch_class(bool synth, bool blue, const RingArgs &a, const RingFft
     launch_range<S, P2>(false, blue, false, a, plans, hplans, nplans, btab, st);
     return;
   }
-  // plans are sorted by decreasing n: rings with n > mmax fold directly
+  // plans are sorted by decreasing n: rings with n > mmax fold directly,
+  // those with 2n > mmax (in the classes of 2048 entries and more) gather
+  // their two aliases per bin from the phase array, the rest stage their
+  // phases in LDS
   int k = 0;
   while (k < nplans && hplans[k].n > a.mmax) ++k;
+  int k2 = k;
+  if (S >= 2048)
+    while (k2 < nplans && 2 * (int64_t)hplans[k2].n > a.mmax) ++k2;
   launch_range<S, P2>(true, blue, false, a, plans, hplans, k, btab, st);
-  launch_range<S, P2>(true, blue, true, a, plans + k, hplans + k, nplans - k, btab, st);
+  launch_range<S, P2>(true, blue, false, a, plans + k, hplans + k, k2 - k, btab, st, 2);
+  launch_range<S, P2>(true, blue, true, a, plans + k2, hplans + k2, nplans - k2, btab, st);
 }
 
 template <int P2>

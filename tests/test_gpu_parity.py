@@ -150,6 +150,28 @@ def test_ring_fft_classes(product, oracle, gpu, spin, jobtype):
     compare(product, oracle, _fft_class_geom, tri(2100), jobtype, spin, 1, O.DP)
 
 
+def _fft_alias_geom(lib):
+    """Rings of the classes of 2048 entries and more whose phases, at mmax
+    2100, fold with at most two aliases per bin (n <= mmax < 2n): even nph
+    2062 (Bluestein M = 4096), 1990 (Bluestein M = 2048), 2100 = mmax (mixed
+    radices; bin 0 takes m = 0 and m = n) and 2058 (N = 1029, radices 3 and
+    7), odd nph 2025 (radices 3, 5) and 2075 (Bluestein M = 8192)."""
+    nph = np.array([2062, 1990, 2100, 2058, 2025, 2075])
+    theta = np.linspace(0.3, 2.8, len(nph))
+    stride = np.ones(len(nph), np.int64)
+    ofs = np.concatenate([[0], np.cumsum(nph)[:-1]]).astype(np.int64)
+    phi0 = np.array([0.2, 0.0, 0.45, 0.1, 0.0, 0.6])
+    wgt = np.linspace(0.7, 1.3, len(nph))
+    return lib.general_geom(nph, ofs, stride, phi0, theta, wgt)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("dp", [True, False])
+def test_ring_fft_two_alias_fold(product, oracle, gpu, spin, dp):
+    compare(product, oracle, _fft_alias_geom, tri(2100), O.ALM2MAP, spin, 2,
+            O.DP if dp else 0, tol=TOL64 if dp else TOL32)
+
+
 def _long_ring_geom(lib):
     """Rings longer than the largest LDS FFT class (8192 complex values):
     nph 16388 (N = 8194 = 2*17*241, Bluestein length 16384), 32768 (N =
