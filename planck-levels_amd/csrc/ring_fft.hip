@@ -874,19 +874,24 @@ __global__ void __launch_bounds__((kSynThreads<S, BLUE, STAGED>), (kMinBlocks<S,
 // ===========================================================================
 // analysis: ring -> phase
 // ===========================================================================
+// threads per workgroup: 1024 for the 8192 Bluestein class, as in synthesis
+// (115 VGPRs with the thread index opaque per component)
+template <int S, bool BLUE>
+constexpr int kAnaThreads = (S == 8192 && BLUE) ? 1024 : kThreads<S>;
+
 template <int S, bool BLUE, int P2, typename T>
-__global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>)) k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
+__global__ void __launch_bounds__((kAnaThreads<S, BLUE>), (kMinBlocks<S, BLUE, false, P2>)) k_ring2phase_fft(RingArgs a, const RingFftPlan *plans,
                                                           int nplans, int comps_per_block, int,
                                                           const double2 *btab)
 {
-  constexpr int NT = kThreads<S>, E = S / NT;
+  constexpr int NT = kAnaThreads<S, BLUE>, E = S / NT;
   extern __shared__ double2 smem[];
   const Lds buf{smem};
   __shared__ double2 nyq;
   const int pi = pair_plan(blockIdx.x);
   if (pi >= nplans) return;
   const RingFftPlan &P = plans[pi];
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
   const int slot = P.ring >> 1, hemi = P.ring & 1;
   const RingDev r = a.ring[P.ring];
   const int n = P.n, N = P.N, mmax = a.mmax;
@@ -912,6 +917,8 @@ __global__ void __launch_bounds__(kThreads<S>, (kMinBlocks<S, BLUE, false, P2>))
   const int c0 = blockIdx.y * comps_per_block;
   const int c1 = min(c0 + comps_per_block, a.ncomp);
   for (int comp = c0; comp < c1; ++comp) {
+    // keeps tid-derived addresses out of the loop-carried registers
+    if constexpr (NT == 1024) asm volatile("" : "+v"(tid));
     const void *mp = a.maps.p[comp];
     // pixel pairs as one vector access: unit stride, pair start aligned
     const bool pairs = even && pair_aligned<T>(mp, r.ofs - a.map_base, r.stride);
@@ -1107,8 +1114,8 @@ static void launch_range(bool synth, bool blue, bool staged, const RingArgs &a,
   } else {
     if constexpr (P2 > 0) {
       if (blue) {
-        if (a.dp) go(k_ring2phase_fft<S, true, P2, double>);
-        else go(k_ring2phase_fft<S, true, P2, float>);
+        if (a.dp) go(k_ring2phase_fft<S, true, P2, double>, kAnaThreads<S, true>);
+        else go(k_ring2phase_fft<S, true, P2, float>, kAnaThreads<S, true>);
       }
     }
     if (!blue) {
