@@ -331,12 +331,42 @@ __global__ void __launch_bounds__(256) k_alm_gather_split(AlmArgs a, int ntiles)
   if (r0 >= np) return;
   const int nr = min(64, np - r0), ncomp = min(16, nc - c0);
   const int lb = o0 + 2 * r0;
-  for (int e = threadIdx.x; e < 128 * 16; e += 256) {
-    const int i = e & 127, cc = e >> 7, l = lb + i, comp = c0 + cc;
-    double2 v = make_double2(0., 0.);
-    if (i < 2 * nr && l <= a.lmax && comp < nc) v = gather_value<T>(a, m, l, comp);
-    tile[i][2 * cc] = (float)v.x;
-    tile[i][2 * cc + 1] = (float)v.y;
+  {
+    // a thread's tile row i (one l) and its components cc = tid/128 + 2u:
+    // the scale and offset once (as gather_value forms them), then the
+    // loads of all 8 components in flight together
+    const int i = threadIdx.x & 127, l = lb + i;
+    const bool lok = i < 2 * nr && l <= a.lmax;
+    double f = 0.;
+    int64_t idx = 0;
+    if (lok) {
+      const bool realonly = a.packed && m == 0;
+      f = a.norm_l[l];
+      if (a.lsc) f *= a.lsc[tri_ofs(m, a.lmax) + l - m];
+      if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrt2;
+      int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
+      if (!a.packed) ofs *= 2;
+      if (!realonly) stride *= 2;
+      idx = ofs + (int64_t)l * stride - a.alm_base;
+    }
+    double re[8], im[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int comp = c0 + (threadIdx.x >> 7) + 2 * u;
+      re[u] = 0.;
+      im[u] = 0.;
+      if (lok && comp < nc) {
+        const void *p = a.alms.p[comp];
+        re[u] = ld<T>(p, idx);
+        if (m != 0) im[u] = ld<T>(p, idx + 1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cc = (threadIdx.x >> 7) + 2 * u;
+      tile[i][2 * cc] = (float)(re[u] * f);
+      tile[i][2 * cc + 1] = (float)(im[u] * f);
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 128 * 32; e += 256) {
