@@ -20,12 +20,12 @@
 #include <algorithm>
 #include <hip/hip_runtime.h>
 #include <type_traits>
+#include "alm_finish.hpp"
 #include "kernels.hpp"
 #include "sharp_lowlevel.h"
 
 namespace shp {
 
-constexpr double kSqrtHalf = 0.707106781186547572737310929369;
 constexpr double kSqrt2 = 1.414213562373095145474621858739;
 
 // e^{2 pi i k / n} for 0 <= k < n
@@ -39,15 +39,6 @@ __device__ inline double2 unit_root(int64_t k, int n)
 __device__ inline double2 cmul(double2 a, double2 b)
 {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-template <typename T> __device__ inline double ld(const void *p, int64_t i)
-{
-  return (double)static_cast<const T *>(p)[i];
-}
-template <typename T> __device__ inline void st(void *p, int64_t i, double v)
-{
-  static_cast<T *>(p)[i] = (T)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -444,74 +435,12 @@ __global__ void __launch_bounds__(256) k_alm_scatter(AlmArgs a)
   const int l = m + (int)(row - tri_ofs(m, a.lmax));
   double2 x;
   if (a.part) {
-    const int64_t gstride = a.part_rows * a.part_ld;
-    const int64_t o = (row - a.part_row0) * a.part_ld + 2 * (comp - a.part_c0);
-    auto part_at = [&](auto f64, int g) {
-      if constexpr (decltype(f64)::value)
-        return *reinterpret_cast<const double2 *>(static_cast<const double *>(a.part) + o +
-                                                  g * gstride);
-      else {
-        const float2 v = *reinterpret_cast<const float2 *>(static_cast<const float *>(a.part) +
-                                                           o + g * gstride);
-        return make_double2(v.x, v.y);
-      }
-    };
-    // the groups that reach m (gmax[g] >= m), added in group order.  When
-    // gmax does not decrease along the groups (ring pairs ordered from a
-    // pole towards the equator, as in every iso-latitude grid) they are the
-    // groups from the first one that reaches m on: their loads are issued
-    // four at a time, not one after the other behind each test
-    int glo = 0;
-    bool mono = true;
-    for (int g = 0; g < a.ngroups; ++g) {
-      const int gm = a.gmax[g];
-      glo += gm < m;
-      if (g > 0) mono = mono && gm >= a.gmax[g - 1];
-    }
-    double re = 0., im = 0.;
-    auto sum = [&](auto f64) {
-      if (mono) {
-        // (indices past the last group read the last group again, unused)
-        for (int g0 = glo; g0 < a.ngroups; g0 += 4) {
-          double2 v[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = part_at(f64, min(g0 + i, a.ngroups - 1));
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (g0 + i < a.ngroups) { re += v[i].x; im += v[i].y; }
-        }
-      } else {
-        for (int g = 0; g < a.ngroups; ++g) {
-          if (a.gmax[g] < m) continue;
-          const double2 v = part_at(f64, g);
-          re += v.x;
-          im += v.y;
-        }
-      }
-    };
-    if (a.part_f64) sum(std::true_type());
-    else sum(std::false_type());
-    x = make_double2(re, im);
+    const GroupRange gr = group_range(a, m);
+    x = a.part_f64 ? part_sum<true>(a, gr, m, row, comp) : part_sum<false>(a, gr, m, row, comp);
   } else {
     x = a.almtmp[row * a.ncomp + comp];
   }
-  const bool realonly = a.packed && m == 0;
-  int64_t ofs = a.mvstart_by_m[m], stride = a.stride;
-  if (!a.packed) ofs *= 2;
-  if (!realonly) stride *= 2;
-  const int64_t idx = ofs + (int64_t)l * stride - a.alm_base;
-  void *p = a.alms.p[comp];
-  double f = a.norm_l[l];
-  if (a.lsc) f *= a.lsc[row];
-  if (m == 0 && (a.flags & SHARP_REAL_HARMONICS)) f *= kSqrtHalf;
-  const bool add = (a.flags & SHARP_ADD) != 0;
-  st<T>(p, idx, (add ? ld<T>(p, idx) : 0.) + x.x * f);
-  if (realonly) return;
-  if (m == 0) {
-    if (!add) st<T>(p, idx + 1, 0.);
-  } else {
-    st<T>(p, idx + 1, (add ? ld<T>(p, idx + 1) : 0.) + x.y * f);
-  }
+  alm_store<T>(a, m, l, row, comp, x);
 }
 
 void launch_alm_gather(const AlmArgs &a, hipStream_t s)
