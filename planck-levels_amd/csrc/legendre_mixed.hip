@@ -1610,8 +1610,14 @@ __global__ void __launch_bounds__(64 * MB) k_msynth_spin(LegArgs a, int nsb)
     msynth_spin_tile<NCB, DERIV, CT>(a, mb * MB + wave_y(), sb, cst[CT ? wave_y() : 0]);
 }
 
+// minimum waves per SIMD: the FP32/BF16 spin-0 analysis with one column block
+// uses 36 KB of LDS per workgroup, so four workgroups fit a CU when it keeps
+// to 128 VGPRs (the BF16 form needs 134 unbounded: 3 per CU, 11% slower)
+template <typename T, int NCB>
+constexpr int kAnal0MinWaves = (sizeof(T) == 4 && NCB == 1) ? 4 : 1;
+
 template <typename T, int NCB, int CT>
-__global__ void __launch_bounds__(64 * kMixWaves0) k_manal_spin0(LegArgs a)
+__global__ void __launch_bounds__(64 * kMixWaves0, (kAnal0MinWaves<T, NCB>)) k_manal_spin0(LegArgs a)
 {
   __shared__ __align__(16) T lds[kMixWaves0][kSlice];
   __shared__ double cbuf[2][64];
@@ -1671,7 +1677,7 @@ __global__ void __launch_bounds__(64 * MB) kb_msynth_spin(const LegArgs *__restr
 }
 
 template <typename T, int NCB, int CT, int W>
-__global__ void __launch_bounds__(64 * W) kb_manal_spin0(const LegArgs *__restrict__ jobs,
+__global__ void __launch_bounds__(64 * W, (kAnal0MinWaves<T, NCB>)) kb_manal_spin0(const LegArgs *__restrict__ jobs,
                                                          const BatchItem *__restrict__ items,
                                                          const int *__restrict__ xofs, int *__restrict__ ctr)
 {
